@@ -1,0 +1,173 @@
+/*
+ * vlasov1d2v.h -- C ABI of the MI355X (gfx950) 1D2V nodal-DG Vlasov-Poisson
+ * SSP-RK3 path.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Every function returns int: 0 = ok, negative = validation
+ * error (see VL_E_*), positive = a hipError_t.  vl_last_error_string()
+ * describes the most recent failure on the calling thread.
+ *
+ * State layout (the reference's, main/grid.py:268): f is a C-order array of
+ * shape (NX+2, n, NU+2, n, NV+2, n) -- element index before node index on each
+ * axis, one ghost element on each side of every axis, v-line contiguous.
+ * E is (NX+2, n) with periodic ghosts; rho is (NX, n).
+ *
+ * No entry point allocates device memory except vl_geom_create (operator
+ * tables, freed by vl_geom_destroy).  Scratch is caller-provided through
+ * `workspace` (size from vl_workspace_bytes).  A handle may be used from one
+ * host thread at a time; calls on one stream are ordered; no call
+ * synchronises the device, so sequences can be captured in a hipGraph.
+ */
+#ifndef VLASOV1D2V_H
+#define VLASOV1D2V_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VL_OK = 0,
+    VL_E_ARG = -1,        /* bad scalar argument (n, dims, dtype, ...)      */
+    VL_E_NULL = -2,       /* null pointer where one is required             */
+    VL_E_ALIGN = -3,      /* device pointer not aligned to its element size */
+    VL_E_HANDLE = -4,     /* invalid handle                                 */
+    VL_E_WORKSPACE = -5,  /* workspace too small                            */
+    VL_E_NOTFINITE = -6   /* non-finite value detected (host-side checks)   */
+};
+
+/* Storage type of the distribution function. */
+enum {
+    VL_F64 = 0,
+    VL_F32 = 1,
+    VL_BF16 = 2,
+    VL_F16 = 3
+};
+
+/* Geometry and operator tables of one run.  All pointers are HOST pointers to
+ * FP64 data; vl_geom_create copies them to device memory.
+ *
+ * Replaces the setup objects Basis1D (main/basis.py:87-228: up, xi, weights),
+ * Grid1D/Grid3D (main/grid.py:10-103: arr, J, wave_numbers, grid_phases,
+ * spectral_transform), DGFlux.__init__ (main/fluxes.py:14-57:
+ * flux_coefficients) and Elliptic (main/elliptic.py:10-22,147-148:
+ * poisson_coefficient, magnetic_field). */
+typedef struct vl_geom_desc {
+    int32_t n;                 /* nodes per axis, 2..10                          */
+    int32_t nx, nu, nv;        /* interior elements per axis (>= 1)              */
+    int32_t n_modes;           /* M = number of Fourier modes (main/grid.py:46)  */
+    int32_t storage;           /* VL_F64 / VL_F32 / VL_BF16 / VL_F16             */
+    double jx, ju, jv;         /* element Jacobians 2/dx (main/grid.py:23)       */
+    double acc;                /* flux_coefficients (main/fluxes.py:53)          */
+    double b_field;            /* magnetic_field (main/elliptic.py:147)          */
+    double poisson_coef;       /* poisson_coefficient (main/elliptic.py:22)      */
+    double x_dx, x_length;     /* dx and length of the x grid (main/grid.py:82)  */
+    const double *up;          /* n*n, up[i*n+j]  (main/basis.py:176-188)        */
+    const double *xi;          /* n*2, xi[i*2+k]  (main/basis.py:110)            */
+    const double *weights;     /* n Lobatto weights (main/basis.py:31-57)        */
+    const double *x_nodes;     /* (nx+2)*n x coordinates (main/grid.py:55-70)   */
+    const double *u_nodes;     /* (nu+2)*n                                       */
+    const double *v_nodes;     /* (nv+2)*n                                       */
+    const double *wave_numbers;/* M  (main/grid.py:48)                           */
+    const double *transform;   /* nx*n*M complex, interleaved re,im
+                                  (Grid1D.spectral_transform, main/basis.py:204)  */
+    const double *phases;      /* M*nx*n complex, interleaved re,im
+                                  (Grid1D.grid_phases, main/grid.py:50)           */
+} vl_geom_desc;
+
+typedef struct vl_geom *vl_geom_t;
+
+/* Create / destroy a geometry handle (setup only; allocates the device copies
+ * of the operator tables). */
+int vl_geom_create(const vl_geom_desc *desc, vl_geom_t *out);
+int vl_geom_destroy(vl_geom_t h);
+
+/* Update the physics coefficients of a handle (host-side, no device work):
+ * flux_coefficients (DGFlux, main/fluxes.py:53), magnetic_field
+ * (Elliptic.set_magnetic_field, main/elliptic.py:147-148) and
+ * poisson_coefficient (Elliptic.__init__, main/elliptic.py:22). */
+int vl_set_physics(vl_geom_t h, double acc, double b_field, double poisson_coef);
+
+/* Bytes of device workspace the calls below need for this geometry. */
+int vl_workspace_bytes(vl_geom_t h, size_t *bytes);
+
+/* Number of elements of one state array: (nx+2)*n*(nu+2)*n*(nv+2)*n. */
+int vl_state_elems(vl_geom_t h, size_t *elems);
+
+/* out = semi-discrete RHS of f (FP64), interior entries only; ghost entries of
+ * `out` are not written.  f's ghost elements are used as given.  E is the
+ * ghosted field (nx+2)*n.
+ * Replaces DGFlux.semi_discrete_rhs (main/fluxes.py:59-207). */
+int vl_rhs(vl_geom_t h, const double *f, const double *E, double *out, void *stream);
+
+/* rho[ex,ix] = sum_{eu,iu,ev,iv} f[interior] * w_iu w_iv / (Ju Jv), FP64.
+ * Replaces Distribution.moment_zero / moment_zero_of_arr (main/grid.py:303-321). */
+int vl_moment0(vl_geom_t h, const double *f, double *rho, void *workspace, void *stream);
+
+/* Spectral Poisson solve with the given charge density (nx*n):
+ * E (nx+2)*n with periodic ghosts, phi (nx*n, may be NULL).  The workspace
+ * holds the DFT coefficients.
+ * Replaces Elliptic.poisson2 (main/elliptic.py:117-145) with
+ * Grid1D.fourier_basis / sum_fourier (main/grid.py:78-88). */
+int vl_poisson(vl_geom_t h, const double *charge_density, double *E, double *phi,
+               void *workspace, void *stream);
+
+/* energy[0] = 0.5 * sum_interior w_ix E^2.
+ * Replaces Elliptic.electric_energy (main/elliptic.py:150-151). */
+int vl_field_energy(vl_geom_t h, const double *E, double *energy, void *stream);
+
+/* out[0] = max |acc * E| over the ghosted field (the device part of
+ * get_max_speeds, main/timestep.py:235-242). */
+int vl_max_abs_acc_e(vl_geom_t h, const double *E, double *out, void *stream);
+
+/* Periodic x-ghost copy of a state array, in place (storage dtype):
+ * f[0] = f[nx], f[nx+1] = f[1].
+ * Replaces Distribution.swap_ghost_cells_of_array (main/grid.py:187-189). */
+int vl_swap_ghosts(vl_geom_t h, void *f, void *stream);
+
+/* One full Shu-Osher SSP-RK3 step of Stepper.nonlinear_ssp_rk
+ * (main/timestep.py:141-187) with a fixed dt:
+ *   x-ghost swap of f; rho(f) -> E; f1 = f + dt L(f);
+ *   rho(f1) -> E; f2 = 3/4 f + 1/4 f1 + 1/4 dt L(f1);
+ *   rho(f2) -> E; f = 1/3 f + 2/3 f2 + 2/3 dt L(f2).
+ * f, f1, f2 are state arrays of the handle's storage dtype; f1/f2 need only
+ * their interior (ghosts are never read: x-neighbours are taken
+ * periodically, u/v ghost neighbours from f, whose u/v ghost planes stay
+ * frozen as in the reference).  On return E (nx+2)*n holds the field solved
+ * from f2 and phi (nx*n, may be NULL) its potential -- the reference's
+ * elliptic state after a step -- and energy_out[0] (device, may be NULL)
+ * = 0.5 sum w E^2 of that field (the value main/timestep.py:102 appends).
+ * flags: VL_STEP_RHO_VALID -- the workspace already holds the moment partials
+ * of f from the previous vl_rk3_step on the same f (skips one read of f).
+ * For VL_F32/VL_BF16/VL_F16 storage, f holds the deviation from the
+ * x-independent background `f0` (nu+2)*n*(nv+2)*n FP64 table set with
+ * vl_set_background; the moment, Poisson solve and energy stay FP64. */
+enum { VL_STEP_RHO_VALID = 1 };
+int vl_rk3_step(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi,
+                double dt, double *energy_out, void *workspace, int flags, void *stream);
+
+/* Optional timing hook (no counterpart in the reference, whose only timing
+ * is a printed wall clock, main/timestep.py:117,139): when `events` is
+ * non-NULL, every later vl_rk3_step on this handle records events[2k] just
+ * before and events[2k+1] just after its k-th stage kernel (k = 0, 1, 2) on
+ * the call's stream.  `events` points to 6 hipEvent_t created by the caller
+ * with timing enabled; the array is read at each call, so the caller may
+ * change its contents between calls.  NULL disables the hook. */
+int vl_set_stage_events(vl_geom_t h, void **events);
+
+/* Background table for the reduced-precision storage forms:
+ * f_total = f0(u,v) + stored deviation.  f0 is a host FP64 array of
+ * (nu+2)*n*(nv+2)*n values.  Only used when storage != VL_F64. */
+int vl_set_background(vl_geom_t h, const double *f0_host);
+
+/* Description of the last failure on this thread (static storage). */
+const char *vl_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VLASOV1D2V_H */

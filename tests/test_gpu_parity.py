@@ -1,0 +1,107 @@
+"""GPU parity of the HIP path against the NumPy oracle on seeded inputs.
+
+Tolerances (FP64 throughout): single operators 1e-13 relative (max-norm);
+multi-step runs 1e-12 relative L2 on f and on the field-energy series.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import interior, make_pair, random_state, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+RHS_CASES = [(n, 3, 4, 5) for n in range(2, 11)] + [(9, 2, 3, 7), (8, 5, 2, 9), (4, 1, 1, 1)]
+
+
+@pytest.mark.parametrize("n,nx,nu,nv", RHS_CASES)
+def test_rhs_random_state(cuda, n, nx, nu, nv):
+    """vl_rhs vs DGFlux.semi_discrete_rhs on a random ghosted f and random E."""
+    import torch
+    case, run = make_pair(n, nx, nu, nv)
+    f = random_state((nx + 2, n, nu + 2, n, nv + 2, n), seed=n * 7 + nx)
+    E = np.random.default_rng(3).standard_normal((nx + 2, n))
+    case.elliptic.electric_field = E.copy()
+    ref = case.rhs(f)
+    run.elliptic.electric_field = torch.from_numpy(E).to(cuda)
+    out = run.flux.semi_discrete_rhs(torch.from_numpy(f).to(cuda), run.elliptic, run.basis, run.grids)
+    got = out.cpu().numpy()
+    assert rel_max(interior(got), interior(ref)) < 1e-13
+    # ghost entries of the output are zero
+    mask = np.ones(got.shape, bool)
+    mask[1:-1, :, 1:-1, :, 1:-1, :] = False
+    assert np.all(got[mask] == 0)
+
+
+@pytest.mark.parametrize("b_field,acc", [(0.1, -1.0), (-0.3, 1.0), (0.0, -1.0)])
+def test_rhs_physical_state(cuda, b_field, acc):
+    """RHS of the ring initial condition with the setup field, both signs of B and acc."""
+    import torch
+    case, run = make_pair(6, 4, 6, 6, b_field=b_field, acc=acc)
+    ref = case.rhs()
+    got = run.flux.semi_discrete_rhs(run.dist.arr, run.elliptic, run.basis, run.grids).cpu().numpy()
+    assert rel_max(interior(got), interior(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("n,nx,nu,nv", [(9, 4, 5, 7), (8, 16, 16, 16), (3, 2, 30, 31)])
+def test_moment(cuda, n, nx, nu, nv):
+    import torch
+    case, run = make_pair(n, nx, nu, nv)
+    f = random_state((nx + 2, n, nu + 2, n, nv + 2, n), seed=11) + 3.0
+    ref = case.dist.moment_zero_of_arr(f)
+    got = run.dist.moment_zero_of_arr(torch.from_numpy(f).to(cuda)).cpu().numpy()
+    assert rel_max(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("n,nx", [(8, 16), (9, 64), (5, 48), (2, 3)])
+def test_poisson_and_energy(cuda, n, nx):
+    import torch
+    case, run = make_pair(n, nx, 2, 2)
+    cd = np.random.default_rng(nx).standard_normal((nx, n))
+    case.elliptic.poisson2(cd, case.grids.x)
+    run.elliptic.poisson2(torch.from_numpy(cd).to(cuda), run.grids.x, run.basis.b1)
+    assert rel_max(run.elliptic.electric_field.cpu().numpy(), case.elliptic.electric_field) < 1e-13
+    assert rel_max(run.elliptic.potential.cpu().numpy(), case.elliptic.potential) < 1e-13
+    e_ref = case.elliptic.electric_energy(case.grids.x)
+    e_got = float(run.elliptic.electric_energy(run.grids.x).item())
+    assert abs(e_got - e_ref) <= 1e-13 * abs(e_ref)
+
+
+def test_swap_ghosts(cuda):
+    import torch
+    case, run = make_pair(4, 5, 3, 3)
+    f = random_state((7, 4, 5, 4, 5, 4), seed=5)
+    case.dist.arr = f.copy()
+    case.dist.swap_ghost_cells_of_array()
+    run.dist.arr = torch.from_numpy(f).to(cuda)
+    run.dist.swap_ghost_cells_of_array()
+    assert np.array_equal(run.dist.arr.cpu().numpy(), case.dist.arr)
+
+
+@pytest.mark.parametrize("n,nx,nu,nv,steps,dt", [(4, 4, 6, 6, 5, 0.01), (9, 3, 4, 5, 3, 0.002),
+                                                  (6, 8, 10, 9, 4, 0.005)])
+def test_rk3_steps(cuda, n, nx, nu, nv, steps, dt):
+    """Several fused SSP-RK3 steps vs the oracle stepper: f and energy series."""
+    case, run = make_pair(n, nx, nu, nv, dt=dt)
+    st_ref = case.run(steps, dt)
+    st = run.run(steps)
+    f_got = run.dist.arr.cpu().numpy()
+    assert rel_l2(interior(f_got), interior(case.dist.arr)) < 1e-12
+    assert rel_l2(st.field_energy, st_ref.field_energy) < 1e-12
+    assert len(st.field_energy) == steps + 1
+    # x ghosts hold the periodic copy taken at the start of the last step, as in the reference
+    assert rel_l2(f_got[0], case.dist.arr[0]) < 1e-12
+    # the elliptic state after the loop is the field of the last stage-3 input
+    assert rel_max(run.elliptic.electric_field.cpu().numpy(), case.elliptic.electric_field) < 1e-11
+
+
+def test_unfused_step_matches_fused(cuda):
+    import torch
+    _, run_a = make_pair(5, 4, 5, 6, dt=0.01)
+    _, run_b = make_pair(5, 4, 5, 6, dt=0.01)
+    run_a.step()
+    run_b.stepper.nonlinear_ssp_rk_unfused(run_b.dist, run_b.basis, run_b.elliptic, run_b.grids, run_b.flux)
+    torch.cuda.synchronize()
+    a = run_a.dist.arr.cpu().numpy()
+    b = run_b.dist.arr.cpu().numpy()
+    assert rel_l2(interior(a), interior(b)) < 1e-14

@@ -1,0 +1,100 @@
+"""CPU tests of the product's host-side setup code and of the C-ABI library
+(symbols only; no device calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import rel_max
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("n", range(2, 11))
+def test_basis_matches_oracle(n):
+    from vlasov1d2v_amd.basis import Basis1D
+    b = Basis1D(n)
+    o = oracle.Basis1D(n)
+    assert np.allclose(b.nodes, o.nodes, rtol=0, atol=4e-16)
+    assert np.allclose(b.weights, o.weights, rtol=0, atol=4e-16)
+    assert np.max(np.abs(b.up - o.up)) < 5e-13 * np.max(np.abs(o.up))
+    assert np.max(np.abs(b.xi - o.xi)) < 5e-13 * np.max(np.abs(o.xi))
+    assert np.max(np.abs(b.vandermonde_inverse - o.vandermonde_inverse)) < 1e-13
+    # both zero the same entries of up (threshold 1e-10, main/basis.py:186)
+    assert np.array_equal(b.up == 0, o.up == 0)
+
+
+@pytest.mark.parametrize("n,nx", [(8, 16), (9, 64), (6, 48)])
+def test_spectral_tables_match_oracle(n, nx):
+    from vlasov1d2v_amd.basis import Basis1D
+    from vlasov1d2v_amd.grid import Grid1D
+    L = 2 * np.pi / 0.1
+    g = Grid1D(-L / 2, L / 2, nx, Basis1D(n), spectrum=True)
+    o = oracle.Grid1D(-L / 2, L / 2, nx, oracle.Basis1D(n), spectrum=True)
+    assert g.nyquist_number == o.nyquist_number
+    assert np.array_equal(g.wave_numbers, o.wave_numbers)
+    assert np.array_equal(g.arr, o.arr)
+    assert np.max(np.abs(g.spectral_transform - o.spectral_transform)) < 1e-13
+    assert np.max(np.abs(g.grid_phases - o.grid_phases)) < 1e-15
+
+
+def test_mode_count_float_expression():
+    """M = 2*floor(2.5 L // dx) - 1 in float arithmetic (main/grid.py:46): 119, not 120, at nx=48."""
+    from vlasov1d2v_amd.basis import Basis1D
+    from vlasov1d2v_amd.grid import Grid1D
+    for k0 in (0.1, 0.05, 0.2):
+        L = 2 * np.pi / k0
+        g = Grid1D(-L / 2, L / 2, 48, Basis1D(3), spectrum=True)
+        assert g.nyquist_number == 2.5 * g.length // g.dx
+    L = 2 * np.pi / 0.1
+    assert Grid1D(-L / 2, L / 2, 48, Basis1D(3), spectrum=True).nyquist_number == 119.0
+    assert Grid1D(-L / 2, L / 2, 64, Basis1D(3), spectrum=True).nyquist_number == 160.0
+
+
+def test_ring_ic_matches_oracle():
+    from vlasov1d2v_amd.basis import Basis3D
+    from vlasov1d2v_amd.grid import Distribution, Grid3D
+    L = 2 * np.pi / 0.1
+    lows, highs = np.array([-L / 2, -8.5, -8.5]), np.array([L / 2, 8.5, 8.5])
+    g = Grid3D(Basis3D([5, 5, 5]), lows, highs, [3, 4, 5])
+    d = Distribution(1.0, 6, [3, 4, 5], [5, 5, 5])
+    ring = d.ring(g)
+    case = oracle.Case(n=5, nx=3, nu=4, nv=5)
+    ref = oracle.ring_distribution(case.grids, 1.0, 6)
+    assert rel_max(ring[None, None], ref) < 1e-15
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "vlasov1d2v.h")).read()
+    return sorted(set(re.findall(r"\b(vl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    from vlasov1d2v_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = _header_symbols()
+    assert set(syms) == set(_lib.EXPORTS)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_library_loads_and_rejects_bad_geometry():
+    """Validation paths return negative codes without touching a device."""
+    from vlasov1d2v_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    d = _lib.VlGeomDesc()
+    d.n = 11
+    h = ctypes.c_void_p()
+    assert lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"n=11" in lib.vl_last_error_string()
+    d.n, d.nx, d.nu, d.nv = 4, 0, 1, 1
+    assert lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert lib.vl_rhs(None, None, None, None, None) == -4
+    assert lib.vl_geom_create(None, ctypes.byref(h)) == -2

@@ -1,0 +1,173 @@
+"""ctypes binding of ``libvlasov1d2v.so`` (declared in include/vlasov1d2v.h).
+
+The library is loaded from this package directory only.  A missing library is
+an error: there is no CPU fallback for any device entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvlasov1d2v.so")
+
+VL_F64, VL_F32, VL_BF16, VL_F16 = 0, 1, 2, 3
+VL_STEP_RHO_VALID = 1
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTS = (
+    "vl_geom_create", "vl_geom_destroy", "vl_workspace_bytes", "vl_state_elems",
+    "vl_rhs", "vl_moment0", "vl_poisson", "vl_field_energy", "vl_max_abs_acc_e",
+    "vl_swap_ghosts", "vl_rk3_step", "vl_set_background", "vl_set_physics", "vl_set_stage_events",
+    "vl_last_error_string",
+)
+
+
+class VlGeomDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("nx", ctypes.c_int32), ("nu", ctypes.c_int32),
+        ("nv", ctypes.c_int32), ("n_modes", ctypes.c_int32), ("storage", ctypes.c_int32),
+        ("jx", ctypes.c_double), ("ju", ctypes.c_double), ("jv", ctypes.c_double),
+        ("acc", ctypes.c_double), ("b_field", ctypes.c_double), ("poisson_coef", ctypes.c_double),
+        ("x_dx", ctypes.c_double), ("x_length", ctypes.c_double),
+        ("up", ctypes.c_void_p), ("xi", ctypes.c_void_p), ("weights", ctypes.c_void_p),
+        ("x_nodes", ctypes.c_void_p), ("u_nodes", ctypes.c_void_p), ("v_nodes", ctypes.c_void_p),
+        ("wave_numbers", ctypes.c_void_p), ("transform", ctypes.c_void_p), ("phases", ctypes.c_void_p),
+    ]
+
+
+class VlasovError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise VlasovError(
+            f"{path} is missing: build it with `python -m vlasov1d2v_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    vp, i32, f64, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    sig = {
+        "vl_geom_create": [ctypes.POINTER(VlGeomDesc), ctypes.POINTER(vp)],
+        "vl_geom_destroy": [vp],
+        "vl_workspace_bytes": [vp, ctypes.POINTER(sz)],
+        "vl_state_elems": [vp, ctypes.POINTER(sz)],
+        "vl_rhs": [vp, vp, vp, vp, vp],
+        "vl_moment0": [vp, vp, vp, vp, vp],
+        "vl_poisson": [vp, vp, vp, vp, vp, vp],
+        "vl_field_energy": [vp, vp, vp, vp],
+        "vl_max_abs_acc_e": [vp, vp, vp, vp],
+        "vl_swap_ghosts": [vp, vp, vp],
+        "vl_rk3_step": [vp, vp, vp, vp, vp, vp, f64, vp, vp, i32, vp],
+        "vl_set_background": [vp, vp],
+        "vl_set_physics": [vp, f64, f64, f64],
+        "vl_set_stage_events": [vp, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.vl_last_error_string.argtypes = []
+    lib.vl_last_error_string.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().vl_last_error_string().decode(errors="replace")
+        raise VlasovError(f"{what} failed (code {rc}): {msg}")
+
+
+def ptr(t):
+    """Device (or host) address of a tensor / array, or None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class Geometry:
+    """Owner of one ``vl_geom_t`` handle (operator tables on the device)."""
+
+    def __init__(self, n, nx, nu, nv, up, xi, weights, x_nodes, u_nodes, v_nodes,
+                 jx, ju, jv, x_dx, x_length, wave_numbers, transform, phases,
+                 acc=-1.0, b_field=0.0, poisson_coef=-1.0, storage=VL_F64):
+        lib = load()
+        self._keep = []
+
+        def arr(a, dtype=np.float64):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        tr = np.ascontiguousarray(transform, dtype=np.complex128)
+        ph = np.ascontiguousarray(phases, dtype=np.complex128)
+        d = VlGeomDesc()
+        d.n, d.nx, d.nu, d.nv = int(n), int(nx), int(nu), int(nv)
+        d.n_modes = int(len(wave_numbers))
+        d.storage = int(storage)
+        d.jx, d.ju, d.jv = float(jx), float(ju), float(jv)
+        d.acc, d.b_field, d.poisson_coef = float(acc), float(b_field), float(poisson_coef)
+        d.x_dx, d.x_length = float(x_dx), float(x_length)
+        d.up = arr(up)
+        d.xi = arr(xi)
+        d.weights = arr(weights)
+        d.x_nodes = arr(x_nodes)
+        d.u_nodes = arr(u_nodes)
+        d.v_nodes = arr(v_nodes)
+        d.wave_numbers = arr(wave_numbers)
+        d.transform = arr(tr.view(np.float64))
+        d.phases = arr(ph.view(np.float64))
+        h = ctypes.c_void_p()
+        check(lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h)), "vl_geom_create")
+        self._keep = []
+        self.handle = h
+        self.n, self.nx, self.nu, self.nv = int(n), int(nx), int(nu), int(nv)
+        self.storage = storage
+        self.acc, self.b_field, self.poisson_coef = float(acc), float(b_field), float(poisson_coef)
+        b = ctypes.c_size_t()
+        check(lib.vl_workspace_bytes(h, ctypes.byref(b)), "vl_workspace_bytes")
+        self.workspace_bytes = int(b.value)
+        e = ctypes.c_size_t()
+        check(lib.vl_state_elems(h, ctypes.byref(e)), "vl_state_elems")
+        self.state_elems = int(e.value)
+        self._ws = None
+
+    def set_physics(self, acc, b_field, poisson_coef):
+        acc, b_field, poisson_coef = float(acc), float(b_field), float(poisson_coef)
+        if (acc, b_field, poisson_coef) != (self.acc, self.b_field, self.poisson_coef):
+            check(load().vl_set_physics(self.handle, acc, b_field, poisson_coef), "vl_set_physics")
+            self.acc, self.b_field, self.poisson_coef = acc, b_field, poisson_coef
+
+    def workspace(self, device):
+        """Device workspace owned by this geometry (allocated once)."""
+        import torch
+        if self._ws is None or self._ws.device != torch.device(device):
+            self._ws = torch.empty(max(self.workspace_bytes, 256), dtype=torch.uint8, device=device)
+        return self._ws
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().vl_geom_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

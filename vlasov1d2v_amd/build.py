@@ -1,0 +1,53 @@
+"""Build the gfx950 HIP library ``libvlasov1d2v.so`` in-tree with hipcc.
+
+Usage: ``python -m vlasov1d2v_amd.build`` (or ``__graft_entry__.build()``).
+The shared library is written next to this file so that it travels with the
+source tree; it is never installed into site-packages.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB_NAME = "libvlasov1d2v.so"
+LIB_PATH = os.path.join(HERE, LIB_NAME)
+SOURCES = ["vlasov.hip"]
+ARCH = "gfx950"
+
+
+def hipcc():
+    cand = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(cand):
+        raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    return cand
+
+
+def compile_cmd(out=LIB_PATH, extra=()):
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+            # packed FP32 instructions are kept out of the device code
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument",
+            "-I", INCLUDE, "-I", CSRC,
+            *extra,
+            "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
+
+
+def build(verbose=True):
+    tmp = LIB_PATH + ".tmp"
+    cmd = compile_cmd(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed with exit code {r.returncode}")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build()

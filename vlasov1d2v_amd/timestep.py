@@ -1,0 +1,183 @@
+"""Shu-Osher SSP-RK3 stepping on the GPU: drop-in for the reference's
+``Stepper`` (main/timestep.py:42-242, nonlinear path).
+
+``nonlinear_ssp_rk`` runs one fused step through ``vl_rk3_step`` (7 kernel
+launches, no host synchronisation).  ``nonlinear_ssp_rk_unfused`` composes
+the per-operator entry points in the reference's order and is kept for
+cross-checking.  With a fixed ``dt`` the loop never synchronises and can be
+captured in a HIP graph; with ``dt=None`` it follows the reference's CFL rule
+(main/timestep.py:184,231-242), which reads max|acc E| back every step.
+"""
+import time as _time
+
+import numpy as np
+
+from . import _lib
+from .grid import _check_state
+
+# Shu-Osher rows [a_fn, a_prev, a_dtL] (main/timestep.py:35-39)
+NONLINEAR_SSP_RK = {
+    2: [[1 / 2, 1 / 2, 1 / 2]],
+    3: [[3 / 4, 1 / 4, 1 / 4],
+        [1 / 3, 2 / 3, 2 / 3]],
+}
+
+# Cockburn-Shu Courant numbers [time_order][space_order - 1] (main/timestep.py:24-33)
+COURANT_NUMBERS = {
+    1: [1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0],
+    2: [1.0, 0.333],
+    3: [1.256, 0.409, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033],
+}
+
+
+def get_max_speeds(grids, elliptic, acc):
+    """|max speeds| per axis (main/timestep.py:235-242); one device->host read."""
+    max_acc_e = float(elliptic.max_abs_acc_e(grids.x, acc).item())
+    b = elliptic.magnetic_field
+    return np.absolute(np.array([grids.u.arr_max,
+                                 max_acc_e + abs(acc * b * grids.v.arr_max),
+                                 acc * abs(b * grids.u.arr_max)]))
+
+
+class Stepper:
+    def __init__(self, time_order, space_order, write_time, final_time, linear=False, dt=None):
+        if linear:
+            raise NotImplementedError("the linear SSP-RK update (main/timestep.py:189-229) is not on this path")
+        if time_order != 3:
+            raise NotImplementedError("the fused GPU step implements the SSP-RK3 row only")
+        self.time_order = int(time_order)
+        self.space_order = int(space_order)
+        self.coefficients = np.array(NONLINEAR_SSP_RK[self.time_order])
+        # The reference's table stops at 9 nodes (IndexError there for 10);
+        # here a missing entry only matters for the adaptive (CFL) time step.
+        row = COURANT_NUMBERS[self.time_order]
+        self.courant = row[self.space_order - 1] if self.space_order - 1 < len(row) else None
+        if self.courant is None and dt is None:
+            raise ValueError(f"no Courant number for {self.space_order} nodes: pass a fixed dt")
+        self.time = 0.0
+        self.fixed_dt = dt
+        self.dt = dt
+        self.steps_counter = 0
+        self.write_counter = 1
+        self.write_time = write_time
+        self.final_time = final_time
+        self.time_array = np.array([self.time])
+        self.field_energy = np.array([])
+        self._stages = None
+        self._rho_token = None
+        self.elapsed = None
+
+    # -- helpers -----------------------------------------------------------
+    def _stage_buffers(self, arr):
+        import torch
+        if (self._stages is None or self._stages[0].shape != arr.shape
+                or self._stages[0].device != arr.device):
+            self._stages = (torch.empty_like(arr), torch.empty_like(arr))
+        return self._stages
+
+    def adapt_time_step(self, max_speeds, dx, du, dv):     # main/timestep.py:231-232
+        self.dt = self.courant / ((max_speeds[0] / dx) + (max_speeds[1] / du) + (max_speeds[2] / dv))
+
+    def _acc(self, dg_flux, refs):
+        if refs is not None and hasattr(refs, "electron_acceleration_multiplier"):
+            return refs.electron_acceleration_multiplier
+        return dg_flux.acceleration_coefficient
+
+    def _adapt(self, grids, elliptic, dg_flux, refs):
+        self.adapt_time_step(get_max_speeds(grids, elliptic, self._acc(dg_flux, refs)),
+                             grids.x.dx, grids.u.dx, grids.v.dx)
+
+    # -- one step ------------------------------------------------------------
+    def nonlinear_ssp_rk(self, distribution, basis, elliptic, grids, dg_flux, refs=None, energy_out=None):
+        """One SSP-RK3 step (main/timestep.py:141-187), fused on the GPU.
+
+        On return ``elliptic.electric_field``/``potential`` hold the field of
+        the third stage's input (as in the reference) and, when given,
+        ``energy_out`` (a 0-d/1-element GPU tensor) its field energy.
+        """
+        import torch
+        arr = distribution.arr
+        _check_state(arr, grids.shape, "nonlinear_ssp_rk")
+        g = dg_flux._geom(grids, elliptic)
+        f1, f2 = self._stage_buffers(arr)
+        dev = arr.device
+        if elliptic.electric_field is None or elliptic.electric_field.device != dev:
+            elliptic.electric_field = torch.zeros((grids.x.res_ghosts, grids.x.order), dtype=torch.float64,
+                                                  device=dev)
+        if elliptic.potential is None or elliptic.potential.device != dev:
+            elliptic.potential = torch.zeros((grids.x.res, grids.x.order), dtype=torch.float64, device=dev)
+        ws = g.workspace(dev)
+        token = (arr.data_ptr(), arr._version, ws.data_ptr())
+        flags = _lib.VL_STEP_RHO_VALID if token == self._rho_token else 0
+        if self.dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+        _lib.check(_lib.load().vl_rk3_step(
+            g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
+            _lib.ptr(elliptic.potential), float(self.dt), _lib.ptr(energy_out), _lib.ptr(ws), flags,
+            _lib.stream_ptr()), "vl_rk3_step")
+        self._rho_token = (arr.data_ptr(), arr._version, ws.data_ptr())
+        if self.fixed_dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+
+    def nonlinear_ssp_rk_unfused(self, distribution, basis, elliptic, grids, dg_flux, refs=None):
+        """The same step composed from the per-operator GPU entry points in the
+        reference's order (main/timestep.py:141-187)."""
+        import torch
+        arr = distribution.arr
+        distribution.swap_ghost_cells_of_array()
+        stages = torch.zeros((self.time_order,) + tuple(arr.shape), dtype=arr.dtype, device=arr.device)
+        stages[:, :, :, 0] = arr[:, :, 0]
+        stages[:, :, :, -1] = arr[:, :, -1]
+        stages[:, :, :, :, :, 0] = arr[:, :, :, :, 0]
+        stages[:, :, :, :, :, -1] = arr[:, :, :, :, -1]
+        ng = grids.no_ghost_slice
+        density = distribution.moment_zero()
+        elliptic.poisson2(torch.mean(density) - density, grids.x, basis.b1)
+        stages[(0,) + ng] = arr[ng] + self.dt * dg_flux.semi_discrete_rhs(arr, elliptic, basis, grids)[ng]
+        for i in range(1, self.time_order):
+            stages[i - 1, 0] = stages[i - 1, -2]
+            stages[i - 1, -1] = stages[i - 1, 1]
+            density = distribution.moment_zero_of_arr(stages[i - 1])
+            elliptic.poisson2(torch.mean(density) - density, grids.x, basis.b1)
+            df_dt = dg_flux.semi_discrete_rhs(stages[i - 1].contiguous(), elliptic, basis, grids)[ng]
+            c = self.coefficients[i - 1]
+            stages[(i,) + ng] = c[0] * arr[ng] + c[1] * stages[(i - 1,) + ng] + c[2] * self.dt * df_dt
+        arr[ng] = stages[-1][ng]
+        self._rho_token = None
+
+    # -- time loop -------------------------------------------------------------
+    def main_loop(self, distribution, basis, elliptic, grids, dg_flux, refs=None, save_file=None,
+                  steps=None):
+        """Step until ``final_time`` (or exactly ``steps`` steps) and record the
+        field-energy series as the reference does (main/timestep.py:81-139).
+
+        With a fixed dt the energies stay on the GPU until the loop ends.
+        """
+        import torch
+        t0 = _time.time()
+        dev = distribution.arr.device
+        e0 = elliptic.electric_energy(grids.x)
+        if self.fixed_dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+            self.field_energy = np.append(self.field_energy, float(e0.item()))
+            while (self.time < self.final_time) if steps is None else (self.steps_counter < steps):
+                energy = torch.empty((), dtype=torch.float64, device=dev)
+                self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs, energy_out=energy)
+                self.time += self.dt
+                self.steps_counter += 1
+                self.time_array = np.append(self.time_array, self.time)
+                self.field_energy = np.append(self.field_energy, float(energy.item()))
+        else:
+            if steps is None:
+                steps = int(np.ceil((self.final_time - self.time) / self.dt - 1e-9))
+            energies = torch.empty(steps + 1, dtype=torch.float64, device=dev)
+            energies[0] = e0
+            for k in range(steps):
+                self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
+                                      energy_out=energies[k + 1:k + 2])
+                self.time += self.dt
+                self.steps_counter += 1
+                self.time_array = np.append(self.time_array, self.time)
+            torch.cuda.synchronize(dev)
+            self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
+        self.elapsed = _time.time() - t0
