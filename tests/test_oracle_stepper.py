@@ -1,0 +1,82 @@
+"""Pins of the oracle's SSP-RK3 stepper (main/timestep.py:35-39,141-187) and
+of its initial condition and moment (main/grid.py:244-321)."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import gammaincc
+
+import oracle
+from helpers import interior
+
+
+class _LinearFlux:
+    def __init__(self, lam):
+        self.lam = lam
+
+    def semi_discrete_rhs(self, function, elliptic, basis, grids):
+        return self.lam * function
+
+
+class _NoField:
+    def poisson2(self, *a, **k):
+        pass
+
+
+@pytest.mark.parametrize("z", [-0.5, 0.3, -2.0, 1e-3])
+def test_rk3_amplification_polynomial(z):
+    """With L f = lam f the step multiplies the interior by 1 + z + z^2/2 + z^3/6."""
+    case = oracle.Case(n=3, nx=2, nu=2, nv=2)
+    f0 = case.dist.arr.copy()
+    dt = 0.1
+    st = oracle.Stepper(dt)
+    st.nonlinear_ssp_rk(case.dist, case.basis, _NoField(), case.grids, _LinearFlux(z / dt))
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    assert np.allclose(interior(case.dist.arr), R * interior(f0), rtol=1e-14, atol=0)
+
+
+def test_shu_osher_table():
+    assert oracle.NONLINEAR_SSP_RK[3] == [[3 / 4, 1 / 4, 1 / 4], [1 / 3, 2 / 3, 2 / 3]]
+
+
+def test_velocity_ghosts_stay_frozen():
+    case = oracle.Case(n=4, nx=3, nu=4, nv=4)
+    f0 = case.dist.arr.copy()
+    case.run(3, 0.01)
+    f = case.dist.arr
+    for sl in [(slice(1, -1), slice(None), 0), (slice(1, -1), slice(None), -1)]:
+        assert np.array_equal(f[sl], f0[sl])
+    assert np.array_equal(f[1:-1, :, :, :, 0], f0[1:-1, :, :, :, 0])
+    assert np.array_equal(f[1:-1, :, :, :, -1], f0[1:-1, :, :, :, -1])
+
+
+@pytest.mark.parametrize("j", [1, 6])
+def test_ring_density_integrates_to_one(j):
+    """The quadrature moment of the unmodulated ring equals 1 minus the tail
+    beyond the box, P(j+1, vmax^2/2)-ish bounds; here checked against the
+    incomplete-gamma tail of the disk r < vmax (inside the square box)."""
+    case = oracle.Case(n=8, nx=2, nu=12, nv=12, ring_j=j, amp1=0.0, amps=(0.0, 0.0, 0.0))
+    rho = case.dist.moment_zero()
+    tail_disk = gammaincc(j + 1, 8.5 ** 2 / 2)       # mass outside r = 8.5
+    assert np.all(rho <= 1.0 + 1e-10)
+    assert np.all(np.abs(rho - 1.0) < max(2 * tail_disk, 1e-9))
+    assert np.ptp(rho) < 1e-14                       # x-independent
+
+
+def test_ring_normalisation_factor():
+    # 1 / (2^j 2 pi j!) for vt = 1 (main/grid.py:253)
+    case = oracle.Case(n=3, nx=1, nu=1, nv=1, ring_j=4, amp1=0.0, amps=(0.0, 0.0, 0.0))
+    u = case.grids.u.arr[1, 1]
+    v = case.grids.v.arr[1, 1]
+    r2 = u * u + v * v
+    want = r2 ** 4 * math.exp(-r2 / 2) / (2 ** 4 * 2 * math.pi * math.factorial(4))
+    assert abs(case.dist.arr[1, 0, 1, 1, 1, 1] - want) < 1e-15 * max(want, 1e-300)
+
+
+def test_energy_series_is_lagged():
+    """Entry k (k >= 1) is the energy of the field solved from f2 of step k
+    (main/timestep.py:102 after :172)."""
+    case = oracle.Case(n=4, nx=3, nu=4, nv=4)
+    st = case.run(2, 0.01)
+    assert len(st.field_energy) == 3
+    assert st.field_energy[-1] == case.elliptic.electric_energy(case.grids.x)
