@@ -91,8 +91,22 @@ int vl_geom_destroy(vl_geom_t h);
  * poisson_coefficient (Elliptic.__init__, main/elliptic.py:22). */
 int vl_set_physics(vl_geom_t h, double acc, double b_field, double poisson_coef);
 
-/* Bytes of device workspace the calls below need for this geometry. */
+/* Bytes of device workspace the calls below need for this geometry.  The
+ * workspace holds moment partials, the density, DFT coefficients and a
+ * completion ticket of the field-energy reduction. */
 int vl_workspace_bytes(vl_geom_t h, size_t *bytes);
+
+/* Prepare a freshly allocated workspace (zeroes its ticket word with a small
+ * kernel, so it is safe inside stream capture).  Call once per workspace
+ * before its first use; every entry point leaves the ticket at zero. */
+int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
+
+/* Launch-shape tuning of the stage kernel (no counterpart in the reference).
+ * VL_TUNE_XSEG: x elements each workgroup walks (default: chosen so that the
+ * launch has about 3072 workgroups). */
+enum { VL_TUNE_XSEG = 1 };
+int vl_set_tuning(vl_geom_t h, int key, int value);
+int vl_get_tuning(vl_geom_t h, int key, int *value);
 
 /* Number of elements of one state array: (nx+2)*n*(nu+2)*n*(nv+2)*n. */
 int vl_state_elems(vl_geom_t h, size_t *elems);

@@ -1,0 +1,63 @@
+"""Write compact fingerprints of oracle runs at the two benchmark sizes.
+
+    python tests/golden/make_oracle_fingerprints.py [ring16] [ring64]
+
+These are outputs of this project's NumPy oracle (oracle/), not of the
+reference: the full state is too large to commit, so each fixture keeps the
+field-energy series, the L2 norm of the interior of f and f at 4096 seeded
+random interior positions.  The GPU tests run the same workloads and compare
+against them (tests/test_gpu_workloads.py).
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+
+RUNS = {
+    # name: (n, nx, nu, nv, steps, dt)
+    "ring16": (8, 16, 16, 16, 50, 0.008),
+    "ring64": (9, 64, 50, 50, 2, 0.002),
+}
+SAMPLE_SEED = 1234
+SAMPLES = 4096
+
+
+def sample_index(shape, seed=SAMPLE_SEED, count=SAMPLES):
+    """Seeded flat indices into the interior block of shape `shape` (interior dims)."""
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(int(np.prod(shape)), size=count, replace=False))
+
+
+def main(names):
+    for name in names:
+        n, nx, nu, nv, steps, dt = RUNS[name]
+        t0 = time.time()
+        case = oracle.Case(n=n, nx=nx, nu=nu, nv=nv, seed=0)
+        st = case.run(steps, dt)
+        f = case.dist.arr[1:-1, :, 1:-1, :, 1:-1, :]
+        idx = sample_index(f.shape)
+        out = {
+            "workload": name, "n": n, "elements": [nx, nu, nv], "steps": steps, "dt": dt, "seed": 0,
+            "k0": 0.1, "b_field": 0.1, "ring_j": 6, "amp1": 1e-3, "amps": [1e-6, 1e-6, 1e-6],
+            "phases": [float(p) for p in case.phases],
+            "field_energy": [float(e) for e in st.field_energy],
+            "f_interior_l2": float(np.linalg.norm(f)),
+            "sample_seed": SAMPLE_SEED,
+            "sample_values": [float(v) for v in f.reshape(-1)[idx]],
+            "oracle_seconds": time.time() - t0,
+        }
+        path = os.path.join(HERE, f"{name}_oracle.json")
+        with open(path, "w") as fh:
+            json.dump(out, fh, indent=0)
+        print(f"{name}: {steps} steps in {out['oracle_seconds']:.1f} s -> {path}", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["ring16"])

@@ -19,8 +19,9 @@ EXPORTS = (
     "vl_geom_create", "vl_geom_destroy", "vl_workspace_bytes", "vl_state_elems",
     "vl_rhs", "vl_moment0", "vl_poisson", "vl_field_energy", "vl_max_abs_acc_e",
     "vl_swap_ghosts", "vl_rk3_step", "vl_set_background", "vl_set_physics", "vl_set_stage_events",
-    "vl_last_error_string",
+    "vl_workspace_init", "vl_set_tuning", "vl_get_tuning", "vl_last_error_string",
 )
+VL_TUNE_XSEG = 1
 
 
 class VlGeomDesc(ctypes.Structure):
@@ -69,6 +70,9 @@ def load(path=LIB_PATH):
         "vl_set_background": [vp, vp],
         "vl_set_physics": [vp, f64, f64, f64],
         "vl_set_stage_events": [vp, vp],
+        "vl_workspace_init": [vp, vp, vp],
+        "vl_set_tuning": [vp, i32, i32],
+        "vl_get_tuning": [vp, i32, ctypes.POINTER(i32)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -158,8 +162,17 @@ class Geometry:
         """Device workspace owned by this geometry (allocated once)."""
         import torch
         if self._ws is None or self._ws.device != torch.device(device):
-            self._ws = torch.empty(max(self.workspace_bytes, 256), dtype=torch.uint8, device=device)
+            self._ws = torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device=device)
+            check(load().vl_workspace_init(self.handle, ptr(self._ws), stream_ptr()), "vl_workspace_init")
         return self._ws
+
+    def set_tuning(self, key, value):
+        check(load().vl_set_tuning(self.handle, int(key), int(value)), "vl_set_tuning")
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(load().vl_get_tuning(self.handle, int(key), ctypes.byref(v)), "vl_get_tuning")
+        return int(v.value)
 
     def close(self):
         if getattr(self, "handle", None):

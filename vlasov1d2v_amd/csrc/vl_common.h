@@ -16,12 +16,117 @@ struct OpTables {
     double xi0[kMaxN];             // face lift, left column  (main/basis.py:110)
     double xi1[kMaxN];             // face lift, right column
     double wq[kMaxN * kMaxN];      // moment weights (w_iu w_iv)/(Ju Jv) (main/grid.py:297-301)
+    float upf[kMaxN * kMaxN];      // FP32 copies for the reduced-precision forms
+    float xi0f[kMaxN];
+    float xi1f[kMaxN];
+    // Even-odd split of the centro-antisymmetric volume operator
+    // (up[n-1-i][n-1-j] = -up[i][j]), h = n/2, m = n/2 (middle node, n odd):
+    //   eoP[i][j] = (up[i][j] + up[n-1-i][j]) / 2,  eoQ[i][j] = (up[i][j] - up[n-1-i][j]) / 2
+    //   eoM[i] = up[i][m],  eoR[j] = up[m][j]          (i, j < h)
+    double eoP[(kMaxN / 2) * (kMaxN / 2)], eoQ[(kMaxN / 2) * (kMaxN / 2)], eoM[kMaxN / 2], eoR[kMaxN / 2];
+    float eoPf[(kMaxN / 2) * (kMaxN / 2)], eoQf[(kMaxN / 2) * (kMaxN / 2)], eoMf[kMaxN / 2], eoRf[kMaxN / 2];
 };
+
+// Raw buffer resource over `bytes` bytes from p (gfx9 data-format word).
+// Loads at offsets >= bytes return 0 and stores there are dropped, so
+// invalid slots use kOob instead of a branch.
+constexpr int kOob = 0x7fffffff;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+typedef unsigned int vl_u32x2 __attribute__((ext_vector_type(2)));
+
+// Storage traits of the distribution function: element type E in HBM,
+// compute type C, whether the stored value is the deviation g = (f - f0) *
+// scale from the x-independent background f0 (reduced-precision forms).
+template <int S> struct Store;
+template <> struct Store<0> {          // FP64: f itself
+    using E = double;
+    using C = double;
+    static constexpr bool kDelta = false;
+    __device__ static C bld(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, int off, C v) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(vl_u32x2, v), r, off, 0, 0);
+    }
+    __device__ static C ld(const E *p, int64_t i) { return p[i]; }
+    __device__ static void st(E *p, int64_t i, C v) { p[i] = v; }
+    __device__ static C round(C v) { return v; }
+};
+template <> struct Store<1> {          // FP32 deviation
+    using E = float;
+    using C = float;
+    static constexpr bool kDelta = true;
+    __device__ static C bld(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, int off, C v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, off, 0, 0);
+    }
+    __device__ static C ld(const E *p, int64_t i) { return p[i]; }
+    __device__ static void st(E *p, int64_t i, C v) { p[i] = v; }
+    __device__ static C round(C v) { return v; }
+};
+template <> struct Store<2> {          // BF16 deviation (round to nearest even)
+    using E = uint16_t;
+    using C = float;
+    static constexpr bool kDelta = true;
+    __device__ static C ld(const E *p, int64_t i) { return (float)__builtin_bit_cast(__bf16, p[i]); }
+    __device__ static C bld(__amdgpu_buffer_rsrc_t r, int off) {
+        return (float)__builtin_bit_cast(__bf16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, int off, C v) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, (__bf16)v), r, off, 0, 0);
+    }
+    __device__ static void st(E *p, int64_t i, C v) { p[i] = __builtin_bit_cast(uint16_t, (__bf16)v); }
+    __device__ static C round(C v) { return (float)(__bf16)v; }
+};
+template <> struct Store<3> {          // FP16 deviation, scaled (round to nearest even)
+    using E = uint16_t;
+    using C = float;
+    static constexpr bool kDelta = true;
+    __device__ static C ld(const E *p, int64_t i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
+    __device__ static C bld(__amdgpu_buffer_rsrc_t r, int off) {
+        return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+    }
+    __device__ static void bst(__amdgpu_buffer_rsrc_t r, int off, C v) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, (_Float16)v), r, off, 0, 0);
+    }
+    __device__ static void st(E *p, int64_t i, C v) { p[i] = __builtin_bit_cast(uint16_t, (_Float16)v); }
+    __device__ static C round(C v) { return (float)(_Float16)v; }
+};
+
+// The tables as seen through the constant address space (scalar loads).
+using ConstOpTables = const __attribute__((address_space(4))) OpTables;
+
+template <typename C> __device__ __forceinline__ C op_up(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_up<double>(ConstOpTables &t, int k) { return t.up[k]; }
+template <> __device__ __forceinline__ float op_up<float>(ConstOpTables &t, int k) { return t.upf[k]; }
+template <typename C> __device__ __forceinline__ C op_xi0(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_xi0<double>(ConstOpTables &t, int k) { return t.xi0[k]; }
+template <> __device__ __forceinline__ float op_xi0<float>(ConstOpTables &t, int k) { return t.xi0f[k]; }
+template <typename C> __device__ __forceinline__ C op_eoP(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_eoP<double>(ConstOpTables &t, int k) { return t.eoP[k]; }
+template <> __device__ __forceinline__ float op_eoP<float>(ConstOpTables &t, int k) { return t.eoPf[k]; }
+template <typename C> __device__ __forceinline__ C op_eoQ(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_eoQ<double>(ConstOpTables &t, int k) { return t.eoQ[k]; }
+template <> __device__ __forceinline__ float op_eoQ<float>(ConstOpTables &t, int k) { return t.eoQf[k]; }
+template <typename C> __device__ __forceinline__ C op_eoM(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_eoM<double>(ConstOpTables &t, int k) { return t.eoM[k]; }
+template <> __device__ __forceinline__ float op_eoM<float>(ConstOpTables &t, int k) { return t.eoMf[k]; }
+template <typename C> __device__ __forceinline__ C op_eoR(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_eoR<double>(ConstOpTables &t, int k) { return t.eoR[k]; }
+template <> __device__ __forceinline__ float op_eoR<float>(ConstOpTables &t, int k) { return t.eoRf[k]; }
+template <typename C> __device__ __forceinline__ C op_xi1(ConstOpTables &t, int k);
+template <> __device__ __forceinline__ double op_xi1<double>(ConstOpTables &t, int k) { return t.xi1[k]; }
+template <> __device__ __forceinline__ float op_xi1<float>(ConstOpTables &t, int k) { return t.xi1f[k]; }
 
 // Geometry / stage scalars of one tile-kernel launch.
 struct StageArgs {
     int nx, nu, nv;                // interior elements
     int nchunk;                    // v-chunks per (ex, eu)
+    int xseg;                      // x elements per pencil segment (stage kernel)
     int periodic_x;                // 1: x-neighbours taken periodically from the interior
     int write_partials;            // 1: emit moment partials of the written values
     double jx, ju, jv;             // Jacobians 2/dx
@@ -29,6 +134,9 @@ struct StageArgs {
     double c0, c1, c2dt;           // out = (c0*fn + c1*fin) + c2dt*L
     int64_t sx, sxe, su, sue;      // strides (elements) of ix, ex, iu, eu; iv stride 1, ev stride n
     double2 *partials;             // [(ex-1)*tiles_per_ex + tile][ix] (hi, lo)
+    // reduced-precision forms: stored g = (f - f0) * scale
+    const double *bg;              // [3][(nu+2)*n][(nv+2)*n]: G+, G-, Hv background tables
+    double scale, inv_scale;       // power of two
 };
 
 // Map a hardware workgroup id to a logical tile so that consecutive logical

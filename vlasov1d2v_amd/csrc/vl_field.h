@@ -1,6 +1,8 @@
 // FP64 field-side kernels: deterministic moment reduction, spectral Poisson
 // solve (high-order nodal DFT), field energy, max |acc E|, periodic x-ghost
-// copy.  All of them are small (O(nx n M) work) and latency-bound.
+// copy.  All of them are small (O(nx n M) work) and latency-bound; they are
+// spread over several workgroups so that the 2x nx*n*M complex tables are
+// streamed by many CUs.
 //
 // Reference semantics (crewsdw/Vlasov1D2V):
 //   moment_zero_of_arr      main/grid.py:319-321
@@ -17,62 +19,18 @@
 namespace vl {
 
 constexpr int kFieldBlock = 1024;
+constexpr int kDftModes = 16;     // modes per workgroup of the forward DFT
+constexpr int kInvNodes = 64;     // nodes per workgroup of the inverse sum
+constexpr int kInvGroups = 4;     // mode groups per node in the inverse sum
 
-// rho[(ex-1)*n + ix] = sum over the tiles of x-element ex of partials, in a
-// fixed order (thread-strided double-double sums, then a fixed tree).
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__restrict__ partials,
-                                                            int n, int tiles_per_ex,
-                                                            double *__restrict__ rho) {
-    __shared__ double sh[kBlock];
-    __shared__ double sl[kBlock];
-    const int ex0 = blockIdx.x;   // 0-based interior x element
-    const int tid = threadIdx.x;
-    for (int ix = 0; ix < n; ++ix) {
-        DD acc;
-        for (int t = tid; t < tiles_per_ex; t += kBlock) {
-            const double2 p = partials[((int64_t)ex0 * tiles_per_ex + t) * n + ix];
-            DD d;
-            d.hi = p.x;
-            d.lo = p.y;
-            acc.add(d);
-        }
-        sh[tid] = acc.hi;
-        sl[tid] = acc.lo;
-        __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {
-            if (tid < w) {
-                DD a, b;
-                a.hi = sh[tid];
-                a.lo = sl[tid];
-                b.hi = sh[tid + w];
-                b.lo = sl[tid + w];
-                a.add(b);
-                sh[tid] = a.hi;
-                sl[tid] = a.lo;
-            }
-            __syncthreads();
-        }
-        if (tid == 0) rho[ex0 * n + ix] = sh[0] + sl[0];
-        __syncthreads();
-    }
-}
-
-struct FieldArgs {
-    int nx, n, M;
-    int from_rho;            // 1: charge density = mean(rho) - rho; 0: given
-    double poisson_coef;
-    double dx, length;
-    double acc;
-};
-
-// Block-wide double-double sum (kFieldBlock threads); returns the rounded
-// value to every thread.
+// Block-wide double-double sum over blockDim.x (a power of two) threads;
+// returns the rounded value to every thread.
 __device__ __forceinline__ double block_sum_dd(DD v, double *sh, double *sl) {
     const int tid = threadIdx.x;
     sh[tid] = v.hi;
     sl[tid] = v.lo;
     __syncthreads();
-    for (int w = kFieldBlock / 2; w > 0; w >>= 1) {
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if (tid < w) {
             DD a, b;
             a.hi = sh[tid];
@@ -94,7 +52,7 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
     const int tid = threadIdx.x;
     sh[tid] = v;
     __syncthreads();
-    for (int w = kFieldBlock / 2; w > 0; w >>= 1) {
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if (tid < w) sh[tid] = fmax(sh[tid], sh[tid + w]);
         __syncthreads();
     }
@@ -103,46 +61,132 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
     return r;
 }
 
-// Spectral Poisson solve + energy (+ max|acc E|) in one workgroup.
-//   cd   = from_rho ? mean(rho) - rho : rho                 (nx*n)
-//   c_m  = (sum_k (poisson_coef*cd_k) T[k][m]) * dx / length
-//   Ehat = -c/(i k), phihat = -c/k^2, k = 0 -> 0
-//   E[1:-1] = Re sum_m Ehat_m phases[m][k]; periodic ghosts
-//   energy = 0.5 * sum w_j E^2
-__global__ __launch_bounds__(kFieldBlock) void k_field(
-    const double *__restrict__ rho_or_cd, const double2 *__restrict__ T,   // [k][m]
-    const double2 *__restrict__ phases,                                   // [m][k]
-    const double *__restrict__ kwave, const double *__restrict__ wts,     // M, n
-    double *__restrict__ cd_ws, double2 *__restrict__ coef_ws,            // nx*n, 2*M
-    double *__restrict__ E, double *__restrict__ phi, double *__restrict__ energy,
-    double *__restrict__ maxabs, FieldArgs a) {
-    __shared__ double sh[kFieldBlock];
-    __shared__ double sl[kFieldBlock];
+// rho[(ex-1)*n + ix] = sum over the tiles of x-element ex of the moment
+// partials, in a fixed order: thread t owns tiles t, t+256, ... (double-double),
+// then a fixed tree over the 256 threads.  One workgroup per x element.
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__restrict__ partials,
+                                                            int tiles_per_ex, double *__restrict__ rho) {
+    __shared__ double sh[N][kBlock];
+    __shared__ double sl[N][kBlock];
+    const int ex0 = blockIdx.x;
     const int tid = threadIdx.x;
-    const int K = a.nx * a.n;
-
-    // 1. charge density
-    if (a.from_rho) {
-        DD s;
-        for (int k = tid; k < K; k += kFieldBlock) s.add(rho_or_cd[k]);
-        const double mean = block_sum_dd(s, sh, sl) / (double)K;
-        for (int k = tid; k < K; k += kFieldBlock) cd_ws[k] = mean - rho_or_cd[k];
-    } else {
-        for (int k = tid; k < K; k += kFieldBlock) cd_ws[k] = rho_or_cd[k];
+    DD acc[N];
+    const double2 *base = partials + (int64_t)ex0 * tiles_per_ex * N;
+    for (int t = tid; t < tiles_per_ex; t += kBlock) {
+        double2 p[N];
+#pragma unroll
+        for (int ix = 0; ix < N; ++ix) p[ix] = base[(int64_t)t * N + ix];
+#pragma unroll
+        for (int ix = 0; ix < N; ++ix) {
+            DD d;
+            d.hi = p[ix].x;
+            d.lo = p[ix].y;
+            acc[ix].add(d);
+        }
+    }
+#pragma unroll
+    for (int ix = 0; ix < N; ++ix) {
+        sh[ix][tid] = acc[ix].hi;
+        sl[ix][tid] = acc[ix].lo;
     }
     __syncthreads();
-
-    // 2. forward nodal DFT and spectral solve
-    for (int m = tid; m < a.M; m += kFieldBlock) {
-        double re = 0.0, im = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double r = a.poisson_coef * cd_ws[k];
-            const double2 t = T[(int64_t)k * a.M + m];
-            re += r * t.x;
-            im += r * t.y;
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        for (int idx = tid; idx < N * w; idx += kBlock) {
+            const int ix = idx / w;
+            const int j = idx - ix * w;
+            DD a, b;
+            a.hi = sh[ix][j];
+            a.lo = sl[ix][j];
+            b.hi = sh[ix][j + w];
+            b.lo = sl[ix][j + w];
+            a.add(b);
+            sh[ix][j] = a.hi;
+            sl[ix][j] = a.lo;
         }
-        re = re * a.dx / a.length;
-        im = im * a.dx / a.length;
+        __syncthreads();
+    }
+    if (tid < N) rho[ex0 * N + tid] = sh[tid][0] + sl[tid][0];
+}
+
+struct FieldArgs {
+    int nx, n, M;
+    int from_rho;            // 1: charge density = mean(rho) - rho; 0: given
+    double poisson_coef;
+    double dx, length;
+    double acc;
+};
+
+// Forward nodal DFT and spectral solve for kDftModes modes per workgroup.
+//   cd   = from_rho ? mean(rho) - rho : rho          (every workgroup forms it
+//          identically: same fixed-order double-double sum)
+//   c_m  = (sum_k (poisson_coef*cd_k) T[k][m]) * dx / length
+//   Ehat = -c/(i k), phihat = -c/k^2, k = 0 -> 0     (main/elliptic.py:124-128)
+__global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict__ rho_or_cd,
+                                                        const double2 *__restrict__ T,   // [k][m]
+                                                        const double *__restrict__ kwave,
+                                                        double2 *__restrict__ coef,      // [2][M]
+                                                        FieldArgs a) {
+    extern __shared__ double dyn[];
+    double *rhs = dyn;                                  // K values
+    __shared__ double sh[kBlock];
+    __shared__ double sl[kBlock];
+    const int tid = threadIdx.x;
+    const int K = a.nx * a.n;
+    double mean = 0.0;
+    if (a.from_rho) {
+        DD s;
+        for (int k = tid; k < K; k += kBlock) s.add(rho_or_cd[k]);
+        mean = block_sum_dd(s, sh, sl) / (double)K;
+    }
+    for (int k = tid; k < K; k += kBlock) {
+        const double cd = a.from_rho ? mean - rho_or_cd[k] : rho_or_cd[k];
+        rhs[k] = a.poisson_coef * cd;
+    }
+    __syncthreads();
+    // thread (group g, mode offset mm): k = g, g + G, ...
+    constexpr int G = kBlock / kDftModes;
+    const int mm = tid % kDftModes;
+    const int g = tid / kDftModes;
+    const int m = blockIdx.x * kDftModes + mm;
+    double re = 0.0, im = 0.0;
+    if (m < a.M) {
+        int k = g;
+        for (; k + 3 * G < K; k += 4 * G) {
+            const double2 t0 = T[(int64_t)k * a.M + m];
+            const double2 t1 = T[(int64_t)(k + G) * a.M + m];
+            const double2 t2 = T[(int64_t)(k + 2 * G) * a.M + m];
+            const double2 t3 = T[(int64_t)(k + 3 * G) * a.M + m];
+            re += rhs[k] * t0.x;
+            im += rhs[k] * t0.y;
+            re += rhs[k + G] * t1.x;
+            im += rhs[k + G] * t1.y;
+            re += rhs[k + 2 * G] * t2.x;
+            im += rhs[k + 2 * G] * t2.y;
+            re += rhs[k + 3 * G] * t3.x;
+            im += rhs[k + 3 * G] * t3.y;
+        }
+        for (; k < K; k += G) {
+            const double2 t0 = T[(int64_t)k * a.M + m];
+            re += rhs[k] * t0.x;
+            im += rhs[k] * t0.y;
+        }
+    }
+    // fixed-order tree over the G groups of each mode
+    double *pr = sh, *pi = sl;
+    pr[tid] = re;
+    pi[tid] = im;
+    __syncthreads();
+    for (int w = G / 2; w > 0; w >>= 1) {
+        if (g < w) {
+            pr[tid] += pr[tid + w * kDftModes];
+            pi[tid] += pi[tid + w * kDftModes];
+        }
+        __syncthreads();
+    }
+    if (g == 0 && m < a.M) {
+        re = pr[tid] * a.dx / a.length;
+        im = pi[tid] * a.dx / a.length;
         const double kw = kwave[m];
         double2 eh = make_double2(0.0, 0.0), ph = make_double2(0.0, 0.0);
         if (kw != 0.0) {
@@ -150,39 +194,109 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(
             const double k2 = kw * kw;
             ph = make_double2(-(re / k2), -(im / k2));
         }
-        coef_ws[m] = eh;
-        coef_ws[a.M + m] = ph;
+        coef[m] = eh;
+        coef[a.M + m] = ph;
     }
-    __syncthreads();
+}
 
-    // 3. inverse sums at the nodes, ghosts, energy, max |acc E|
-    DD en;
-    double mx = 0.0;
-    for (int k = tid; k < K; k += kFieldBlock) {
-        double e = 0.0, p = 0.0;
-        for (int m = 0; m < a.M; ++m) {
+// Inverse sums at the nodes: E[1:-1] = Re sum_m Ehat_m phases[m][k] (and phi),
+// periodic ghosts, and -- when energy/maxabs are requested -- the field energy
+// 0.5 sum w E^2 and max|acc E|, reduced in a fixed order by the last workgroup
+// to finish (ticket counter in the workspace; it returns to zero).
+__global__ __launch_bounds__(kBlock) void k_dft_inverse(
+    const double2 *__restrict__ coef, const double2 *__restrict__ phases,   // [2][M], [m][k]
+    const double *__restrict__ wts, double *__restrict__ E, double *__restrict__ phi,
+    double2 *__restrict__ en_part, double *__restrict__ mx_part, unsigned int *__restrict__ ticket,
+    double *__restrict__ energy, double *__restrict__ maxabs, FieldArgs a) {
+    extern __shared__ double2 cdyn[];                 // 2*M coefficients
+    __shared__ double pe[kBlock], pp[kBlock];
+    __shared__ double sh[kInvNodes], sl[kInvNodes], smx[kInvNodes];
+    __shared__ int is_last;
+    const int tid = threadIdx.x;
+    const int K = a.nx * a.n;
+    const int M = a.M;
+    for (int i = tid; i < 2 * M; i += kBlock) cdyn[i] = coef[i];
+    __syncthreads();
+    const int kk = tid % kInvNodes;
+    const int g = tid / kInvNodes;
+    const int k = blockIdx.x * kInvNodes + kk;
+    double e = 0.0, p = 0.0;
+    if (k < K) {
+        for (int m = g; m < M; m += kInvGroups) {
             const double2 ph = phases[(int64_t)m * K + k];
-            const double2 c = coef_ws[m];
+            const double2 c = cdyn[m];
             e += c.x * ph.x - c.y * ph.y;
             if (phi) {
-                const double2 cp = coef_ws[a.M + m];
+                const double2 cp = cdyn[M + m];
                 p += cp.x * ph.x - cp.y * ph.y;
             }
         }
-        const int el = k / a.n;
-        const int j = k - el * a.n;
-        E[(el + 1) * a.n + j] = e;
-        if (el == a.nx - 1) E[j] = e;                      // E[0] = E[-2]
-        if (el == 0) E[(a.nx + 1) * a.n + j] = e;          // E[-1] = E[1]
-        if (phi) phi[k] = p;
-        en.add(mul_rn(mul_rn(e, e), wts[j]));
-        mx = fmax(mx, fabs(a.acc * e));
     }
-    const double tot = block_sum_dd(en, sh, sl);
-    const double gmx = block_max(mx, sh);
+    pe[tid] = e;
+    pp[tid] = p;
+    __syncthreads();
+    for (int w = kInvGroups / 2; w > 0; w >>= 1) {
+        if (g < w) {
+            pe[tid] += pe[tid + w * kInvNodes];
+            pp[tid] += pp[tid + w * kInvNodes];
+        }
+        __syncthreads();
+    }
+    const bool want_red = energy || maxabs;
+    if (g == 0) {
+        DD en;
+        double mx = 0.0;
+        if (k < K) {
+            e = pe[tid];
+            const int el = k / a.n;
+            const int j = k - el * a.n;
+            E[(el + 1) * a.n + j] = e;
+            if (el == a.nx - 1) E[j] = e;                      // E[0] = E[-2]
+            if (el == 0) E[(a.nx + 1) * a.n + j] = e;          // E[-1] = E[1]
+            if (phi) phi[k] = pp[tid];
+            en.add(mul_rn(mul_rn(e, e), wts[j]));
+            mx = fabs(a.acc * e);
+        }
+        sh[kk] = en.hi;
+        sl[kk] = en.lo;
+        smx[kk] = mx;
+    }
+    if (!want_red) return;
+    __syncthreads();
     if (tid == 0) {
-        if (energy) energy[0] = 0.5 * tot;
-        if (maxabs) maxabs[0] = gmx;
+        DD s;
+        double mx = 0.0;
+        for (int i = 0; i < kInvNodes; ++i) {
+            DD d;
+            d.hi = sh[i];
+            d.lo = sl[i];
+            s.add(d);
+            mx = fmax(mx, smx[i]);
+        }
+        en_part[blockIdx.x] = make_double2(s.hi, s.lo);
+        mx_part[blockIdx.x] = mx;
+        __threadfence();
+        const unsigned int prev = atomicAdd(ticket, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    if (tid == 0) {
+        __threadfence();
+        DD s;
+        double mx = 0.0;
+        const volatile double *vp = reinterpret_cast<const volatile double *>(en_part);
+        const volatile double *vm = mx_part;
+        for (int b = 0; b < (int)gridDim.x; ++b) {
+            DD d;
+            d.hi = vp[2 * b];
+            d.lo = vp[2 * b + 1];
+            s.add(d);
+            mx = fmax(mx, vm[b]);
+        }
+        if (energy) energy[0] = 0.5 * s.value();
+        if (maxabs) maxabs[0] = mx;
+        atomicExch(ticket, 0u);
     }
 }
 
@@ -209,6 +323,13 @@ __global__ __launch_bounds__(kFieldBlock) void k_max_abs_acc_e(const double *__r
     for (int k = threadIdx.x; k < count; k += kFieldBlock) mx = fmax(mx, fabs(acc * E[k]));
     const double g = block_max(mx, sh);
     if (threadIdx.x == 0) out[0] = g;
+}
+
+// Fill a small buffer with zeros (used instead of a memset so that the call
+// sequence stays capturable in a HIP graph).
+__global__ void k_zero_words(unsigned int *__restrict__ p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0u;
 }
 
 // Periodic x-ghost copy, in place: plane 0 <- plane nx, plane nx+1 <- plane 1.

@@ -14,7 +14,9 @@
 #include "vlasov1d2v.h"
 #include "vl_common.h"
 #include "vl_field.h"
-#include "vl_stage_f64.h"
+#include "vl_stage.h"
+
+#include <vector>
 
 using namespace vl;
 
@@ -23,15 +25,20 @@ struct vl_geom {
     double jx, ju, jv, acc, bfield, pcoef, dx, length;
     OpTables tab;
     int TV, nchunk, tiles_per_ex, tiles;
+    int xseg, pencils;     // stage kernel: x elements per pencil segment, pencils launched
     int64_t sx, sxe, su, sue;
     size_t state_elems;
     int device;
     double *d_unodes, *d_vnodes, *d_wts, *d_kwave;
     double2 *d_T, *d_phases;
-    double *d_f0;
+    double *d_bg;          // reduced forms: G+, G-, Hv background tables
+    OpTables *d_tab;       // device copy of tab (read by the stage kernel)
+    std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
+    double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
     void **stage_events;   // optional: 6 hipEvent_t recorded around the stage kernels
     // workspace layout (byte offsets)
-    size_t off_partials, off_rho, off_cd, off_coef, ws_bytes;
+    size_t off_partials, off_rho, off_coef, off_enpart, off_mxpart, off_ticket, ws_bytes;
+    int inv_blocks;
 };
 
 namespace {
@@ -83,6 +90,7 @@ StageArgs make_stage_args(const vl_geom *h) {
     s.nu = h->nu;
     s.nv = h->nv;
     s.nchunk = h->nchunk;
+    s.xseg = h->xseg;
     s.jx = h->jx;
     s.ju = h->ju;
     s.jv = h->jv;
@@ -92,6 +100,9 @@ StageArgs make_stage_args(const vl_geom *h) {
     s.sxe = h->sxe;
     s.su = h->su;
     s.sue = h->sue;
+    s.bg = h->d_bg;
+    s.scale = h->scale;
+    s.inv_scale = h->inv_scale;
     return s;
 }
 
@@ -110,48 +121,81 @@ FieldArgs make_field_args(const vl_geom *h, int from_rho) {
 
 // ---- launch helpers (template dispatch over n) --------------------------
 
-template <int N, int MODE>
-hipError_t launch_stage_n(const vl_geom *h, const double *fin, const double *fgh, const double *fn,
-                          double *fout, const double *E, const StageArgs &s, hipStream_t st) {
-    hipLaunchKernelGGL((k_stage_f64<N, MODE>), dim3(h->tiles), dim3(kBlock), 0, st, fin, fgh, fn, fout,
-                       E, h->d_unodes, h->d_vnodes, s, h->tab);
+template <int N, int MODE, int S>
+hipError_t launch_stage_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                           const double *E, const StageArgs &s, hipStream_t st) {
+    using T = typename Store<S>::E;
+    hipLaunchKernelGGL((k_stage<N, MODE, S>), dim3(h->pencils), dim3(kBlock), 0, st, (const T *)fin,
+                       (const T *)fgh, (const T *)fn, (T *)fout, E, h->d_unodes, h->d_vnodes, s, h->d_tab);
     return hipGetLastError();
 }
 
-template <int MODE>
-hipError_t launch_stage(const vl_geom *h, const double *fin, const double *fgh, const double *fn,
-                        double *fout, const double *E, const StageArgs &s, hipStream_t st) {
+template <int MODE, int S>
+hipError_t launch_stage_s(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                          const double *E, const StageArgs &s, hipStream_t st) {
     switch (h->n) {
-        case 2: return launch_stage_n<2, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 3: return launch_stage_n<3, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 4: return launch_stage_n<4, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 5: return launch_stage_n<5, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 6: return launch_stage_n<6, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 7: return launch_stage_n<7, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 8: return launch_stage_n<8, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 9: return launch_stage_n<9, MODE>(h, fin, fgh, fn, fout, E, s, st);
-        case 10: return launch_stage_n<10, MODE>(h, fin, fgh, fn, fout, E, s, st);
+        case 2: return launch_stage_ns<2, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 3: return launch_stage_ns<3, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 4: return launch_stage_ns<4, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 5: return launch_stage_ns<5, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 6: return launch_stage_ns<6, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 7: return launch_stage_ns<7, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 8: return launch_stage_ns<8, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 9: return launch_stage_ns<9, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+        case 10: return launch_stage_ns<10, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <int N>
-hipError_t launch_moment_n(const vl_geom *h, const double *f, const StageArgs &s, hipStream_t st) {
-    hipLaunchKernelGGL((k_moment_partials_f64<N>), dim3(h->tiles), dim3(kBlock), 0, st, f, s, h->tab);
+// RHS mode exists for FP64 only (the drop-in DGFlux.semi_discrete_rhs)
+template <int MODE>
+hipError_t launch_stage(const vl_geom *h, int storage, const void *fin, const void *fgh, const void *fn,
+                        void *fout, const double *E, const StageArgs &s, hipStream_t st) {
+    switch (storage) {
+        case VL_F64: return launch_stage_s<MODE, VL_F64>(h, fin, fgh, fn, fout, E, s, st);
+        case VL_F32: return launch_stage_s<MODE, VL_F32>(h, fin, fgh, fn, fout, E, s, st);
+        case VL_BF16: return launch_stage_s<MODE, VL_BF16>(h, fin, fgh, fn, fout, E, s, st);
+        case VL_F16: return launch_stage_s<MODE, VL_F16>(h, fin, fgh, fn, fout, E, s, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+template <>
+hipError_t launch_stage<kModeRhs>(const vl_geom *h, int storage, const void *fin, const void *fgh,
+                                  const void *fn, void *fout, const double *E, const StageArgs &s,
+                                  hipStream_t st) {
+    if (storage != VL_F64) return hipErrorInvalidValue;
+    return launch_stage_s<kModeRhs, VL_F64>(h, fin, fgh, fn, fout, E, s, st);
+}
+
+template <int N, int S>
+hipError_t launch_moment_ns(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
+    using T = typename Store<S>::E;
+    hipLaunchKernelGGL((k_moment_partials<N, S>), dim3(h->tiles), dim3(kBlock), 0, st, (const T *)f, s, h->tab);
     return hipGetLastError();
 }
 
-hipError_t launch_moment(const vl_geom *h, const double *f, const StageArgs &s, hipStream_t st) {
+template <int S>
+hipError_t launch_moment_s(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
     switch (h->n) {
-        case 2: return launch_moment_n<2>(h, f, s, st);
-        case 3: return launch_moment_n<3>(h, f, s, st);
-        case 4: return launch_moment_n<4>(h, f, s, st);
-        case 5: return launch_moment_n<5>(h, f, s, st);
-        case 6: return launch_moment_n<6>(h, f, s, st);
-        case 7: return launch_moment_n<7>(h, f, s, st);
-        case 8: return launch_moment_n<8>(h, f, s, st);
-        case 9: return launch_moment_n<9>(h, f, s, st);
-        case 10: return launch_moment_n<10>(h, f, s, st);
+        case 2: return launch_moment_ns<2, S>(h, f, s, st);
+        case 3: return launch_moment_ns<3, S>(h, f, s, st);
+        case 4: return launch_moment_ns<4, S>(h, f, s, st);
+        case 5: return launch_moment_ns<5, S>(h, f, s, st);
+        case 6: return launch_moment_ns<6, S>(h, f, s, st);
+        case 7: return launch_moment_ns<7, S>(h, f, s, st);
+        case 8: return launch_moment_ns<8, S>(h, f, s, st);
+        case 9: return launch_moment_ns<9, S>(h, f, s, st);
+        case 10: return launch_moment_ns<10, S>(h, f, s, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_moment(const vl_geom *h, int storage, const void *f, const StageArgs &s, hipStream_t st) {
+    switch (storage) {
+        case VL_F64: return launch_moment_s<VL_F64>(h, f, s, st);
+        case VL_F32: return launch_moment_s<VL_F32>(h, f, s, st);
+        case VL_BF16: return launch_moment_s<VL_BF16>(h, f, s, st);
+        case VL_F16: return launch_moment_s<VL_F16>(h, f, s, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -162,25 +206,63 @@ double2 *ws_partials(const vl_geom *h, void *ws) {
 double *ws_rho(const vl_geom *h, void *ws) {
     return reinterpret_cast<double *>(static_cast<char *>(ws) + h->off_rho);
 }
-double *ws_cd(const vl_geom *h, void *ws) {
-    return reinterpret_cast<double *>(static_cast<char *>(ws) + h->off_cd);
-}
 double2 *ws_coef(const vl_geom *h, void *ws) {
     return reinterpret_cast<double2 *>(static_cast<char *>(ws) + h->off_coef);
+}
+double2 *ws_enpart(const vl_geom *h, void *ws) {
+    return reinterpret_cast<double2 *>(static_cast<char *>(ws) + h->off_enpart);
+}
+double *ws_mxpart(const vl_geom *h, void *ws) {
+    return reinterpret_cast<double *>(static_cast<char *>(ws) + h->off_mxpart);
+}
+unsigned int *ws_ticket(const vl_geom *h, void *ws) {
+    return reinterpret_cast<unsigned int *>(static_cast<char *>(ws) + h->off_ticket);
+}
+
+template <int N>
+hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, hipStream_t st) {
+    hipLaunchKernelGGL((k_reduce_partials<N>), dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws),
+                       h->tiles_per_ex, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st) {
+    switch (h->n) {
+        case 2: return launch_reduce_n<2>(h, ws, rho, st);
+        case 3: return launch_reduce_n<3>(h, ws, rho, st);
+        case 4: return launch_reduce_n<4>(h, ws, rho, st);
+        case 5: return launch_reduce_n<5>(h, ws, rho, st);
+        case 6: return launch_reduce_n<6>(h, ws, rho, st);
+        case 7: return launch_reduce_n<7>(h, ws, rho, st);
+        case 8: return launch_reduce_n<8>(h, ws, rho, st);
+        case 9: return launch_reduce_n<9>(h, ws, rho, st);
+        case 10: return launch_reduce_n<10>(h, ws, rho, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// spectral solve: forward DFT (M/16 workgroups), inverse sums (K/64 workgroups)
+hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho, void *ws, double *E,
+                        double *phi, double *energy, double *maxabs, hipStream_t st) {
+    const FieldArgs a = make_field_args(h, from_rho);
+    const int K = h->nx * h->n;
+    const int gf = (h->M + kDftModes - 1) / kDftModes;
+    hipLaunchKernelGGL(k_dft_forward, dim3(gf), dim3(kBlock), (size_t)K * sizeof(double), st, rho_or_cd,
+                       h->d_T, h->d_kwave, ws_coef(h, ws), a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dft_inverse, dim3(h->inv_blocks), dim3(kBlock), (size_t)2 * h->M * sizeof(double2),
+                       st, ws_coef(h, ws), h->d_phases, h->d_wts, E, phi, ws_enpart(h, ws), ws_mxpart(h, ws),
+                       ws_ticket(h, ws), energy, maxabs, a);
+    return hipGetLastError();
 }
 
 // partials -> rho (workspace) -> E (+phi, energy)
 hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *phi, double *energy,
                                hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws), h->n,
-                       h->tiles_per_ex, ws_rho(h, ws));
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st);
     if (e != hipSuccess) return e;
-    const FieldArgs a = make_field_args(h, 1);
-    hipLaunchKernelGGL(k_field, dim3(1), dim3(kFieldBlock), 0, st, ws_rho(h, ws), h->d_T, h->d_phases,
-                       h->d_kwave, h->d_wts, ws_cd(h, ws), ws_coef(h, ws), E, phi, energy,
-                       (double *)nullptr, a);
-    return hipGetLastError();
+    return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st);
 }
 
 hipError_t record_stage_event(const vl_geom *h, int idx, hipStream_t st) {
@@ -200,6 +282,58 @@ hipError_t alloc_copy(T **dst, const T *src, size_t count) {
     return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
 }
 
+// Background RHS tables of the reduced forms from f0 (host, FP64):
+//   G+[eu,iu,gv] = Ju (sum_j up[iu][j] f0_j + xi0[iu] f0(eu-1, n-1) - xi1[iu] f0_{n-1})
+//   G-[eu,iu,gv] = Ju (sum_j up[iu][j] f0_j + xi0[iu] f0_0 - xi1[iu] f0(eu+1, 0))
+// so that L_u(f0) = a_u G+ (a_u >= 0) or a_u G- (a_u < 0) on the u-line, and
+//   Hv[eu,iu,gv] = L_v(f0) with a_v = acc (-B u), the upwind choice of
+// main/fluxes.py:181-207 for one v-line.  Ghost rows/columns stay zero.
+int upload_background(vl_geom *h) {
+    const int n = h->n, nu = h->nu, nv = h->nv;
+    const int64_t W = (int64_t)(nv + 2) * n;           // row length (v nodes)
+    const int64_t tbl = (int64_t)(nu + 2) * n * W;
+    std::vector<double> bg((size_t)(3 * tbl), 0.0);
+    const double *f0 = h->f0_host.data();
+    auto F0 = [&](int eu, int iu, int64_t gv) { return f0[((int64_t)eu * n + iu) * W + gv]; };
+    std::vector<double> unodes((size_t)(nu + 2) * n);
+    VL_HIP(hipMemcpy(unodes.data(), h->d_unodes, unodes.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int eu = 1; eu <= nu; ++eu)
+        for (int ev = 1; ev <= nv; ++ev)
+            for (int iv = 0; iv < n; ++iv) {
+                const int64_t gv = (int64_t)ev * n + iv;
+                for (int i = 0; i < n; ++i) {
+                    double vol = 0.0;
+                    for (int j = 0; j < n; ++j) vol += h->tab.up[i * n + j] * F0(eu, j, gv);
+                    const double gp = vol + h->tab.xi0[i] * F0(eu - 1, n - 1, gv) - h->tab.xi1[i] * F0(eu, n - 1, gv);
+                    const double gm = vol + h->tab.xi0[i] * F0(eu, 0, gv) - h->tab.xi1[i] * F0(eu + 1, 0, gv);
+                    const int64_t o = ((int64_t)eu * n + i) * W + gv;
+                    bg[o] = h->ju * gp;
+                    bg[tbl + o] = h->ju * gm;
+                }
+            }
+    for (int eu = 1; eu <= nu; ++eu)
+        for (int iu = 0; iu < n; ++iu) {
+            const double av = h->acc * (-h->bfield * unodes[(size_t)eu * n + iu]);
+            const bool pos = av >= 0.0;
+            const int64_t row = ((int64_t)eu * n + iu) * W;
+            for (int ev = 1; ev <= nv; ++ev) {
+                const double *fr = f0 + row + (int64_t)ev * n;
+                const double fl_nb = f0[row + (int64_t)ev * n - 1];
+                const double fr_nb = f0[row + (int64_t)(ev + 1) * n];
+                for (int i = 0; i < n; ++i) {
+                    double vol = 0.0;
+                    for (int j = 0; j < n; ++j) vol += h->tab.up[i * n + j] * fr[j];
+                    const double face = pos ? (h->tab.xi0[i] * fl_nb - h->tab.xi1[i] * fr[n - 1])
+                                            : (h->tab.xi0[i] * fr[0] - h->tab.xi1[i] * fr_nb);
+                    bg[2 * tbl + row + (int64_t)ev * n + i] = h->jv * av * (vol + face);
+                }
+            }
+        }
+    if (!h->d_bg) VL_HIP(hipMalloc((void **)&h->d_bg, bg.size() * sizeof(double)));
+    VL_HIP(hipMemcpy(h->d_bg, bg.data(), bg.size() * sizeof(double), hipMemcpyHostToDevice));
+    return VL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -213,15 +347,16 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     if (d->nx < 1 || d->nu < 1 || d->nv < 1) return fail(VL_E_ARG, "element counts must be >= 1");
     if (d->n_modes < 1) return fail(VL_E_ARG, "n_modes must be >= 1");
     if (storage_bytes(d->storage) == 0) return fail(VL_E_ARG, "unknown storage %d", d->storage);
-    if (d->storage != VL_F64) return fail(VL_E_ARG, "storage %d not available in this build", d->storage);
     if (!d->up || !d->xi || !d->weights || !d->u_nodes || !d->v_nodes || !d->wave_numbers ||
         !d->transform || !d->phases)
         return fail(VL_E_NULL, "vl_geom_create: null table");
     if (!(d->x_dx > 0) || !(d->x_length > 0)) return fail(VL_E_ARG, "x_dx/x_length must be > 0");
+    // LDS of the spectral kernels: nx*n doubles (forward), 2*M complex (inverse)
+    if ((int64_t)d->nx * d->n > 6144) return fail(VL_E_ARG, "nx*n=%d above 6144", d->nx * d->n);
+    if (d->n_modes > 1536) return fail(VL_E_ARG, "n_modes=%d above 1536", d->n_modes);
 
-    vl_geom *h = new (std::nothrow) vl_geom;
+    vl_geom *h = new (std::nothrow) vl_geom();
     if (!h) return fail(VL_E_ARG, "out of host memory");
-    std::memset(h, 0, sizeof(*h));
     const int n = d->n;
     h->n = n;
     h->nx = d->nx;
@@ -242,6 +377,39 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         h->tab.xi0[i] = d->xi[2 * i];
         h->tab.xi1[i] = d->xi[2 * i + 1];
     }
+    // even-odd split of up (exact: the Lobatto volume operator is
+    // centro-antisymmetric; checked here)
+    {
+        const int H = n / 2, M = n / 2;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                if (h->tab.up[(n - 1 - i) * n + (n - 1 - j)] != -h->tab.up[i * n + j]) {
+                    delete h;
+                    return fail(VL_E_ARG, "volume operator is not centro-antisymmetric at (%d,%d)", i, j);
+                }
+        for (int i = 0; i < H; ++i) {
+            for (int j = 0; j < H; ++j) {
+                h->tab.eoP[i * H + j] = 0.5 * (h->tab.up[i * n + j] + h->tab.up[(n - 1 - i) * n + j]);
+                h->tab.eoQ[i * H + j] = 0.5 * (h->tab.up[i * n + j] - h->tab.up[(n - 1 - i) * n + j]);
+                h->tab.eoPf[i * H + j] = (float)h->tab.eoP[i * H + j];
+                h->tab.eoQf[i * H + j] = (float)h->tab.eoQ[i * H + j];
+            }
+            if (n % 2) {
+                h->tab.eoM[i] = h->tab.up[i * n + M];
+                h->tab.eoR[i] = h->tab.up[M * n + i];
+                h->tab.eoMf[i] = (float)h->tab.eoM[i];
+                h->tab.eoRf[i] = (float)h->tab.eoR[i];
+            }
+        }
+    }
+    for (int i = 0; i < n * n; ++i) h->tab.upf[i] = (float)h->tab.up[i];
+    for (int i = 0; i < n; ++i) {
+        h->tab.xi0f[i] = (float)h->tab.xi0[i];
+        h->tab.xi1f[i] = (float)h->tab.xi1[i];
+    }
+    // FP16 storage keeps g * 2^16 so that the deviation stays in the normal range
+    h->scale = d->storage == VL_F16 ? 65536.0 : 1.0;
+    h->inv_scale = 1.0 / h->scale;
     // moment weights exactly as main/grid.py:297-301 forms them:
     // (w_iu * w_iv) / (Ju * Jv)
     for (int iu = 0; iu < n; ++iu)
@@ -256,25 +424,45 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         return fail(VL_E_ARG, "grid too large");
     }
     h->tiles = (int)tiles;
+    // pencils of the stage kernel: enough x segments for ~4 waves of 3
+    // workgroups on each of 256 CUs (host-tunable with vl_set_tuning)
+    {
+        const int base = d->nu * h->nchunk;
+        int nseg = (3072 + base - 1) / base;
+        nseg = std::max(1, std::min(nseg, d->nx));
+        h->xseg = (d->nx + nseg - 1) / nseg;
+        h->pencils = base * ((d->nx + h->xseg - 1) / h->xseg);
+    }
     h->su = (int64_t)(d->nv + 2) * n;
     h->sue = h->su * n;
     h->sx = h->sue * (d->nu + 2);
     h->sxe = h->sx * n;
     h->state_elems = (size_t)h->sxe * (d->nx + 2);
+    // buffer addressing: one x element of the state must span < 2^31 bytes
+    if (h->sxe * (int64_t)storage_bytes(d->storage) >= 2147483647LL) {
+        delete h;
+        return fail(VL_E_ARG, "one x element of the state exceeds 2^31 bytes");
+    }
 
     size_t off = 0;
     h->off_partials = off;
     off = align_up(off + (size_t)tiles * n * sizeof(double2), 256);
     h->off_rho = off;
     off = align_up(off + (size_t)d->nx * n * sizeof(double), 256);
-    h->off_cd = off;
-    off = align_up(off + (size_t)d->nx * n * sizeof(double), 256);
     h->off_coef = off;
     off = align_up(off + (size_t)2 * h->M * sizeof(double2), 256);
+    h->inv_blocks = (d->nx * n + kInvNodes - 1) / kInvNodes;
+    h->off_enpart = off;
+    off = align_up(off + (size_t)h->inv_blocks * sizeof(double2), 256);
+    h->off_mxpart = off;
+    off = align_up(off + (size_t)h->inv_blocks * sizeof(double), 256);
+    h->off_ticket = off;
+    off = align_up(off + 256, 256);
     h->ws_bytes = off;
 
     hipError_t e = hipGetDevice(&h->device);
     const int K = d->nx * n;
+    if (e == hipSuccess) e = alloc_copy(&h->d_tab, &h->tab, 1);
     if (e == hipSuccess) e = alloc_copy(&h->d_unodes, d->u_nodes, (size_t)(d->nu + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_vnodes, d->v_nodes, (size_t)(d->nv + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_wts, d->weights, (size_t)n);
@@ -293,7 +481,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
 int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
-    void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts, h->d_kwave, h->d_T, h->d_phases, h->d_f0};
+    void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts, h->d_kwave, h->d_T, h->d_phases, h->d_bg, h->d_tab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -304,9 +492,12 @@ int vl_set_physics(vl_geom_t h, double acc, double b_field, double poisson_coef)
     if (int r = check_handle(h)) return r;
     if (!std::isfinite(acc) || !std::isfinite(b_field) || !std::isfinite(poisson_coef))
         return fail(VL_E_NOTFINITE, "vl_set_physics: non-finite coefficient");
+    const bool changed = acc != h->acc || b_field != h->bfield;
     h->acc = acc;
     h->bfield = b_field;
     h->pcoef = poisson_coef;
+    if (changed && !h->f0_host.empty())
+        if (int r = upload_background(h)) return r;
     return VL_OK;
 }
 
@@ -331,7 +522,7 @@ int vl_rhs(vl_geom_t h, const double *f, const double *E, double *out, void *str
     StageArgs s = make_stage_args(h);
     s.periodic_x = 0;
     s.write_partials = 0;
-    VL_HIP(launch_stage<kModeRhs>(h, f, f, nullptr, out, E, s, (hipStream_t)stream));
+    VL_HIP(launch_stage<kModeRhs>(h, VL_F64, f, f, nullptr, out, E, s, (hipStream_t)stream));
     return VL_OK;
 }
 
@@ -342,10 +533,8 @@ int vl_moment0(vl_geom_t h, const double *f, double *rho, void *ws, void *stream
     hipStream_t st = (hipStream_t)stream;
     StageArgs s = make_stage_args(h);
     s.partials = ws_partials(h, ws);
-    VL_HIP(launch_moment(h, f, s, st));
-    hipLaunchKernelGGL(k_reduce_partials, dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws), h->n,
-                       h->tiles_per_ex, rho);
-    VL_HIP(hipGetLastError());
+    VL_HIP(launch_moment(h, VL_F64, f, s, st));
+    VL_HIP(launch_reduce(h, ws, rho, st));
     return VL_OK;
 }
 
@@ -354,11 +543,7 @@ int vl_poisson(vl_geom_t h, const double *cd, double *E, double *phi, void *ws, 
     if (!cd || !E || !ws) return fail(VL_E_NULL, "vl_poisson: null pointer");
     if (!aligned(cd, 8) || !aligned(E, 8) || !aligned(ws, 256) || (phi && !aligned(phi, 8)))
         return fail(VL_E_ALIGN, "vl_poisson: misaligned");
-    const FieldArgs a = make_field_args(h, 0);
-    hipLaunchKernelGGL(k_field, dim3(1), dim3(kFieldBlock), 0, (hipStream_t)stream, cd, h->d_T,
-                       h->d_phases, h->d_kwave, h->d_wts, ws_cd(h, ws), ws_coef(h, ws), E, phi,
-                       (double *)nullptr, (double *)nullptr, a);
-    VL_HIP(hipGetLastError());
+    VL_HIP(launch_field(h, cd, 0, ws, E, phi, nullptr, nullptr, (hipStream_t)stream));
     return VL_OK;
 }
 
@@ -403,25 +588,28 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
                 double *energy_out, void *ws, int flags, void *stream) {
     if (int r = check_handle(h)) return r;
     if (!fv || !f1v || !f2v || !E || !ws) return fail(VL_E_NULL, "vl_rk3_step: null pointer");
-    if (!aligned(fv, 8) || !aligned(f1v, 8) || !aligned(f2v, 8) || !aligned(E, 8) || !aligned(ws, 256))
+    const size_t eb = storage_bytes(h->storage);
+    if (!aligned(fv, eb) || !aligned(f1v, eb) || !aligned(f2v, eb) || !aligned(E, 8) || !aligned(ws, 256))
         return fail(VL_E_ALIGN, "vl_rk3_step: misaligned pointer");
     if (!std::isfinite(dt)) return fail(VL_E_NOTFINITE, "vl_rk3_step: dt not finite");
-    if (h->storage != VL_F64) return fail(VL_E_ARG, "vl_rk3_step: storage not available");
-    double *f = (double *)fv, *f1 = (double *)f1v, *f2 = (double *)f2v;
+    if (h->storage != VL_F64 && !h->d_bg)
+        return fail(VL_E_ARG, "vl_rk3_step: reduced storage needs vl_set_background first");
+    void *f = fv, *f1 = f1v, *f2 = f2v;
+    const int S = h->storage;
     hipStream_t st = (hipStream_t)stream;
     if (int r = vl_swap_ghosts(h, f, stream)) return r;
     StageArgs s = make_stage_args(h);
     s.periodic_x = 1;
     s.write_partials = 1;
     s.partials = ws_partials(h, ws);
-    if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, f, s, st));
+    if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, f, s, st));
     // stage 1: f1 = f + dt L(f)                       (main/timestep.py:153-163)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
     s.c0 = 0.0;
     s.c1 = 1.0;
     s.c2dt = dt;
     VL_HIP(record_stage_event(h, 0, st));
-    VL_HIP(launch_stage<kModeStage1>(h, f, f, nullptr, f1, E, s, st));
+    VL_HIP(launch_stage<kModeStage1>(h, S, f, f, nullptr, f1, E, s, st));
     VL_HIP(record_stage_event(h, 1, st));
     // stage 2: f2 = 3/4 f + 1/4 f1 + 1/4 dt L(f1)     (main/timestep.py:165-182)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
@@ -429,7 +617,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.c1 = 1.0 / 4.0;
     s.c2dt = (1.0 / 4.0) * dt;
     VL_HIP(record_stage_event(h, 2, st));
-    VL_HIP(launch_stage<kModeStage>(h, f1, f, f, f2, E, s, st));
+    VL_HIP(launch_stage<kModeStage>(h, S, f1, f, f, f2, E, s, st));
     VL_HIP(record_stage_event(h, 3, st));
     // stage 3: f = 1/3 f + 2/3 f2 + 2/3 dt L(f2); E, phi, energy of f2's field
     VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st));
@@ -437,9 +625,39 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.c1 = 2.0 / 3.0;
     s.c2dt = (2.0 / 3.0) * dt;
     VL_HIP(record_stage_event(h, 4, st));
-    VL_HIP(launch_stage<kModeStage>(h, f2, f, f, f, E, s, st));
+    VL_HIP(launch_stage<kModeStage>(h, S, f2, f, f, f, E, s, st));
     VL_HIP(record_stage_event(h, 5, st));
     return VL_OK;
+}
+
+int vl_workspace_init(vl_geom_t h, void *ws, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!ws) return fail(VL_E_NULL, "vl_workspace_init: null workspace");
+    if (!aligned(ws, 256)) return fail(VL_E_ALIGN, "vl_workspace_init: misaligned");
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, (hipStream_t)stream, ws_ticket(h, ws), 64);
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_set_tuning(vl_geom_t h, int key, int value) {
+    if (int r = check_handle(h)) return r;
+    if (key == VL_TUNE_XSEG) {
+        if (value < 1) return fail(VL_E_ARG, "xseg must be >= 1");
+        h->xseg = std::min(value, h->nx);
+        h->pencils = h->nu * h->nchunk * ((h->nx + h->xseg - 1) / h->xseg);
+        return VL_OK;
+    }
+    return fail(VL_E_ARG, "unknown tuning key %d", key);
+}
+
+int vl_get_tuning(vl_geom_t h, int key, int *value) {
+    if (int r = check_handle(h)) return r;
+    if (!value) return fail(VL_E_NULL, "vl_get_tuning: null value");
+    if (key == VL_TUNE_XSEG) {
+        *value = h->xseg;
+        return VL_OK;
+    }
+    return fail(VL_E_ARG, "unknown tuning key %d", key);
 }
 
 int vl_set_stage_events(vl_geom_t h, void **events) {
@@ -451,7 +669,13 @@ int vl_set_stage_events(vl_geom_t h, void **events) {
 int vl_set_background(vl_geom_t h, const double *f0_host) {
     if (int r = check_handle(h)) return r;
     if (!f0_host) return fail(VL_E_NULL, "vl_set_background: null pointer");
-    return fail(VL_E_ARG, "vl_set_background: storage not available in this build");
+    if (h->storage == VL_F64) return fail(VL_E_ARG, "vl_set_background: FP64 storage keeps f itself");
+    const size_t cnt = (size_t)(h->nu + 2) * h->n * (h->nv + 2) * h->n;
+    for (size_t i = 0; i < cnt; ++i)
+        if (!std::isfinite(f0_host[i])) return fail(VL_E_NOTFINITE, "vl_set_background: non-finite f0");
+    h->f0_host.assign(f0_host, f0_host + cnt);
+    if (int r = upload_background(h)) return r;
+    return VL_OK;
 }
 
 }  // extern "C"

@@ -15,8 +15,8 @@ class DGFlux:
         self.grid_axis = np.array([0, 2, 4])
         self.sub_element_axis = np.array([1, 3, 5])
 
-    def _geom(self, grids, elliptic):
-        g = grids.geometry()
+    def _geom(self, grids, elliptic, storage=_lib.VL_F64):
+        g = grids.geometry(storage)
         exp = tuple(int(r) for r in self.resolutions)
         got = (grids.x.res_ghosts, grids.u.res_ghosts, grids.v.res_ghosts)
         if exp != got:

@@ -104,26 +104,48 @@ class Grid3D:
         if not (basis.b1.order == basis.b2.order == basis.b3.order):
             raise ValueError("the HIP path needs the same number of nodes on x, u and v")
         self.x._owner = self
+        self._geometries = {}
         self.no_ghost_slice = (slice(1, self.x.res_ghosts - 1), slice(self.x.order),
                                slice(1, self.u.res_ghosts - 1), slice(self.u.order),
                                slice(1, self.v.res_ghosts - 1), slice(self.v.order))
-        self._geometry = None
         self.shape = (self.x.res_ghosts, self.x.order, self.u.res_ghosts, self.u.order,
                       self.v.res_ghosts, self.v.order)
 
-    def geometry(self):
-        """The HIP geometry handle of this phase-space grid (created once)."""
-        if self._geometry is None:
+    def geometry(self, storage=_lib.VL_F64):
+        """The HIP geometry handle of this phase-space grid for one storage
+        type of f (created once per storage type)."""
+        g = self._geometries.get(storage)
+        if g is None:
             b = self.basis.b1
-            self._geometry = _lib.Geometry(
+            g = _lib.Geometry(
                 b.order, self.x.res, self.u.res, self.v.res, b.up, b.xi, b.weights,
                 self.x.arr, self.u.arr, self.v.arr, self.x.J, self.u.J, self.v.J,
                 self.x.dx, self.x.length, self.x.wave_numbers, self.x.spectral_transform,
-                self.x.grid_phases)
-        return self._geometry
+                self.x.grid_phases, storage=storage)
+            self._geometries[storage] = g
+        return g
 
 
-def _check_state(arr, shape, what):
+STORAGE_CODES = {"f64": _lib.VL_F64, "f32": _lib.VL_F32, "bf16": _lib.VL_BF16, "f16": _lib.VL_F16}
+# stored g = (f - f0) * scale for the reduced forms (FP16 keeps g in its normal range)
+STORAGE_SCALE = {"f64": 1.0, "f32": 1.0, "bf16": 1.0, "f16": 65536.0}
+
+
+def storage_dtype(storage):
+    import torch
+    return {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16,
+            "f16": torch.float16}[storage]
+
+
+def storage_of(arr):
+    import torch
+    m = {torch.float64: "f64", torch.float32: "f32", torch.bfloat16: "bf16", torch.float16: "f16"}
+    if arr.dtype not in m:
+        raise TypeError(f"unsupported state dtype {arr.dtype}")
+    return m[arr.dtype]
+
+
+def _check_state(arr, shape, what, allow_reduced=False):
     import torch
     if not isinstance(arr, torch.Tensor):
         raise TypeError(f"{what}: expected a torch tensor on the GPU")
@@ -133,7 +155,7 @@ def _check_state(arr, shape, what):
         raise ValueError(f"{what}: shape {tuple(arr.shape)} != {tuple(shape)}")
     if not arr.is_contiguous():
         raise ValueError(f"{what}: tensor must be contiguous")
-    if arr.dtype != torch.float64:
+    if arr.dtype != torch.float64 and not (allow_reduced and storage_of(arr) != "f64"):
         raise TypeError(f"{what}: expected float64")
 
 
@@ -154,6 +176,9 @@ class Distribution:
         self.quad_weights = None
         self.device = device
         self._grids = None
+        self.storage = "f64"
+        self.scale = 1.0
+        self.f0 = None          # background ring (NU+2, n, NV+2, n) of the reduced forms
 
     def initialize_quad_weights(self, grids):
         """W[eu,iu,ev,iv] = w_iu w_iv / (Ju Jv) (main/grid.py:297-301); host copy."""
@@ -179,10 +204,10 @@ class Distribution:
         """
         import torch
         self._grids = grids
-        ring = self.ring(grids)
+        ring = torch.from_numpy(self.ring(grids)).to(self.device)
         mod = np.ones_like(grids.x.arr) if modulation is None else np.asarray(modulation)
-        host = ring[None, None, :, :, :, :] * mod[:, :, None, None, None, None]
-        self.arr = torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+        mod = torch.from_numpy(np.ascontiguousarray(mod)).to(self.device)
+        self.arr = (ring[None, None, :, :, :, :] * mod[:, :, None, None, None, None]).contiguous()
         if self.quad_weights is None:
             self.initialize_quad_weights(grids)
 
@@ -191,9 +216,43 @@ class Distribution:
         x = grids.x.arr
         k1 = grids.x.k1
         mod = 1.0 + amp1 * np.cos(k1 * x)
+        pert = amp1 * np.cos(k1 * x)
         for m, (a, ph) in enumerate(zip(amps, phases)):
             mod = mod + a * np.cos((m + 2) * k1 * x + ph)
+            pert = pert + a * np.cos((m + 2) * k1 * x + ph)
+        self._pert = pert
         self.initialize_gpu(grids, modulation=mod)
+
+    def to_storage(self, grids, storage):
+        """Switch to a reduced-precision form: the state becomes
+        g = (f - f0(u, v)) * scale in the storage dtype, with f0 the ring and
+        f - f0 = ring * (modulation - 1) formed without cancellation.
+        The background f0 is registered with the HIP geometry."""
+        import torch
+        if storage == "f64":
+            return
+        if getattr(self, "_pert", None) is None:
+            raise RuntimeError("to_storage needs initialize_ring_modulated first")
+        self.storage = storage
+        self.scale = STORAGE_SCALE[storage]
+        ring_h = self.ring(grids)
+        self.f0 = ring_h
+        ring = torch.from_numpy(ring_h).to(self.device)
+        pert = torch.from_numpy(np.ascontiguousarray(self._pert)).to(self.device)
+        g = ring[None, None, :, :, :, :] * pert[:, :, None, None, None, None]
+        self.arr = (g * self.scale).to(storage_dtype(storage)).contiguous()
+        del g
+        geo = grids.geometry(STORAGE_CODES[storage])
+        _lib.check(_lib.load().vl_set_background(geo.handle, _lib.ptr(np.ascontiguousarray(ring_h))),
+                   "vl_set_background")
+
+    def total(self):
+        """f in FP64 on the GPU (f itself, or f0 + g / scale for the reduced forms)."""
+        import torch
+        if self.storage == "f64":
+            return self.arr
+        f0 = torch.from_numpy(self.f0).to(self.arr.device)
+        return f0[None, None] + self.arr.to(torch.float64) / self.scale
 
     def _geom(self):
         if self._grids is None:

@@ -13,7 +13,7 @@ import time as _time
 import numpy as np
 
 from . import _lib
-from .grid import _check_state
+from .grid import STORAGE_CODES, _check_state, storage_of
 
 # Shu-Osher rows [a_fn, a_prev, a_dtL] (main/timestep.py:35-39)
 NONLINEAR_SSP_RK = {
@@ -97,8 +97,8 @@ class Stepper:
         """
         import torch
         arr = distribution.arr
-        _check_state(arr, grids.shape, "nonlinear_ssp_rk")
-        g = dg_flux._geom(grids, elliptic)
+        _check_state(arr, grids.shape, "nonlinear_ssp_rk", allow_reduced=True)
+        g = dg_flux._geom(grids, elliptic, STORAGE_CODES[storage_of(arr)])
         f1, f2 = self._stage_buffers(arr)
         dev = arr.device
         if elliptic.electric_field is None or elliptic.electric_field.device != dev:

@@ -54,7 +54,7 @@ class Run:
 
     def __init__(self, n, nx, nu, nv, k0=0.1, b_field=0.1, ring_j=6, vmax=8.5, vt=1.0, acc=-1.0,
                  poisson=-1.0, amp1=1e-3, amps=(1e-6, 1e-6, 1e-6), seed=0, phases=None, dt=0.002,
-                 device="cuda"):
+                 device="cuda", storage="f64"):
         self.basis = Basis3D(np.array([n, n, n]))
         L = 2.0 * np.pi / k0
         lows = np.array([-L / 2.0, -vmax * vt, -vmax * vt])
@@ -70,12 +70,20 @@ class Run:
         self.flux._geom(self.grids, self.elliptic)
         n0 = self.dist.moment_zero()
         self.elliptic.poisson2(n0 - n0.mean(), self.grids.x, self.basis.b1)   # main/main.py:131
+        self.storage = storage
+        if storage != "f64":
+            self.dist.to_storage(self.grids, storage)
         self.stepper = Stepper(3, n, write_time=1e30, final_time=1e30, dt=dt)
 
     @classmethod
     def from_workload(cls, w, dt=None, device="cuda", **kw):
         return cls(w.n, w.nx, w.nu, w.nv, k0=w.k0, b_field=w.b_field, ring_j=w.ring_j, vmax=w.vmax,
                    seed=w.seed, dt=w.dt if dt is None else dt, device=device, **kw)
+
+    @property
+    def dof(self):
+        g = self.grids
+        return g.x.res * g.u.res * g.v.res * g.x.order ** 3
 
     def step(self, energy_out=None):
         self.stepper.nonlinear_ssp_rk(self.dist, self.basis, self.elliptic, self.grids, self.flux,
