@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ring64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--storage", default="all", choices=["all", "f64", "f32", "bf16", "f16"],
+                    help="storage form(s) to time; the headline value is the FP64 form")
+    ap.add_argument("--layout", default="elem", choices=["elem", "ref"],
+                    help="resident element-major layout (default) or the reference layout")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -73,22 +77,68 @@ def main():
         sys.exit(2)
 
     import torch
-    from vlasov1d2v_amd import CONFIGS, Run
-    from vlasov1d2v_amd import _lib
+    from vlasov1d2v_amd import CONFIGS
+    from vlasov1d2v_amd.grid import STORAGE_CODES
 
     w = CONFIGS[args.workload]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    run = Run.from_workload(w, device=dev)
-    g = run.grids.geometry()
-    K, W = args.steps, args.warmup
+    forms = {}
+    storages = ["f64", "f32", "bf16", "f16"] if args.storage == "all" else [args.storage]
+    for storage in storages:
+        forms[storage] = measure(w, storage, args.steps, args.warmup, args.layout, dev)
+        torch.cuda.empty_cache()
+    head = forms["f64"] if "f64" in forms else forms[storages[0]]
+    N = w.dof
+    s_bytes = head["bytes_per_value"]
+    stage_bytes = [2 * N * s_bytes, 3 * N * s_bytes, 3 * N * s_bytes]   # algorithmic bytes per launch
+    achieved = sum(stage_bytes) / (sum(head["stage_ms"]) / 1e3) / 1e9
+    out = {
+        "metric": "RK3 DOF-updates/s at 116.6M DOF, FP64 vs BF16-store; % HBM roofline",
+        "value": head["dof_updates_per_s"],
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["storage"],
+        "data": "synthetic (seeded ring distribution with density modulation)",
+        "config": {"workload": w.name, "dof": w.dof, "n": w.n, "elements": [w.nx, w.nu, w.nv],
+                   "dt": w.dt, "storage": head["storage"], "layout": args.layout,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stage (fused RHS + stage combination + moment partials), "
+                               + head["storage"],
+                     "stage_ms": head["stage_ms"], "stage_algorithmic_bytes": stage_bytes},
+        "forms": {k: {kk: v[kk] for kk in ("dof_updates_per_s", "ms_per_step", "stage_ms",
+                                           "step_hbm_frac", "finite")} for k, v in forms.items()},
+    }
+    if "bf16" in forms and "f64" in forms:
+        out["bf16_over_f64"] = forms["bf16"]["dof_updates_per_s"] / forms["f64"]["dof_updates_per_s"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+def measure(w, storage, K, W, layout, dev):
+    """Time K SSP-RK3 steps (after W warmup steps) of one storage form."""
+    import ctypes
+    import torch
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+
+    run = Run.from_workload(w, device=dev, storage=storage)
+    g = run.grids.geometry(STORAGE_CODES[storage])
     energies = torch.empty(W + K + 1, dtype=torch.float64, device=dev)
+    if layout == "elem":
+        run.resident_begin()
     for k in range(W):
         run.step(energy_out=energies[k:k + 1])
     torch.cuda.synchronize()
-
-    # per-stage-kernel events (recorded by the library on the launch stream)
-    import ctypes
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(K)]
     for row in evs:
         for e in row:
@@ -96,7 +146,6 @@ def main():
     torch.cuda.synchronize()
     ev_arrays = [(ctypes.c_void_p * 6)(*[e.cuda_event for e in row]) for row in evs]
     lib = _lib.load()
-
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -107,46 +156,17 @@ def main():
     stop.record()
     torch.cuda.synchronize()
     lib.vl_set_stage_events(g.handle, None)
+    if layout == "elem":
+        run.resident_end()
     ms = start.elapsed_time(stop)
-    ms_per_step = ms / K
-    value = w.dof * K / (ms / 1e3)
-
-    stage_ms = [0.0, 0.0, 0.0]
-    for row in evs:
-        for s in range(3):
-            stage_ms[s] += row[2 * s].elapsed_time(row[2 * s + 1])
-    stage_ms = [t / K for t in stage_ms]
-    s_bytes = 8
-    N = w.dof
-    stage_bytes = [2 * N * s_bytes, 3 * N * s_bytes, 3 * N * s_bytes]   # algorithmic bytes per launch
-    achieved = sum(stage_bytes) / (sum(stage_ms) / 1e3) / 1e9
-    fin = torch.isfinite(energies).all().item()
-
-    out = {
-        "metric": "RK3 DOF-updates/s at 116.6M DOF (FP64 storage); % HBM roofline",
-        "value": value,
-        "unit": "DOF-updates/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded ring distribution with density modulation)",
-        "config": {"workload": w.name, "dof": w.dof, "n": w.n, "elements": [w.nx, w.nu, w.nv],
-                   "dt": w.dt, "storage": "f64", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage_f64<9,*> (fused RHS + stage combination + moment partials)",
-                     "stage_ms": stage_ms, "stage_algorithmic_bytes": stage_bytes},
-        "step_hbm_frac": (8 * N * s_bytes) / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
-        "finite": bool(fin),
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w)
-    print(json.dumps(out), flush=True)
+    stage_ms = [sum(row[2 * s].elapsed_time(row[2 * s + 1]) for row in evs) / K for s in range(3)]
+    bpv = {"f64": 8, "f32": 4, "bf16": 2, "f16": 2}[storage]
+    res = {"storage": storage, "bytes_per_value": bpv, "ms_per_step": ms / K,
+           "dof_updates_per_s": w.dof * K / (ms / 1e3), "stage_ms": stage_ms,
+           "step_hbm_frac": 8 * w.dof * bpv / (ms / K / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "finite": bool(torch.isfinite(energies).all().item())}
+    del run
+    return res
 
 
 if __name__ == "__main__":

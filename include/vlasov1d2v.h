@@ -103,13 +103,27 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
 
 /* Launch-shape tuning of the stage kernel (no counterpart in the reference).
  * VL_TUNE_XSEG: x elements each workgroup walks (default: chosen so that the
- * launch has about 3072 workgroups). */
+ * launch has about 13000 workgroups). */
 enum { VL_TUNE_XSEG = 1 };
 int vl_set_tuning(vl_geom_t h, int key, int value);
 int vl_get_tuning(vl_geom_t h, int key, int *value);
 
 /* Number of elements of one state array: (nx+2)*n*(nu+2)*n*(nv+2)*n. */
 int vl_state_elems(vl_geom_t h, size_t *elems);
+
+/* Layouts of a state array (no counterpart in the reference, which has only
+ * the first): VL_LAYOUT_REFERENCE is the C-order layout above;
+ * VL_LAYOUT_ELEMENT_MAJOR keeps the n^3 nodes of each element contiguous
+ * ((ix, iu, iv) inside, elements ordered (ex, eu, ev)) with a pitch of
+ * round_up(n^3, 8) values per element, so that a tile of elements is one
+ * contiguous run.  It is the resident layout of Stepper.main_loop. */
+enum { VL_LAYOUT_REFERENCE = 0, VL_LAYOUT_ELEMENT_MAJOR = 1 };
+int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems);
+
+/* dst = src in the other layout (out of place; element-major padding values
+ * are not written). */
+int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, int dst_layout,
+                      void *stream);
 
 /* out = semi-discrete RHS of f (FP64), interior entries only; ghost entries of
  * `out` are not written.  f's ghost elements are used as given.  E is the
@@ -156,10 +170,12 @@ int vl_swap_ghosts(vl_geom_t h, void *f, void *stream);
  * = 0.5 sum w E^2 of that field (the value main/timestep.py:102 appends).
  * flags: VL_STEP_RHO_VALID -- the workspace already holds the moment partials
  * of f from the previous vl_rk3_step on the same f (skips one read of f).
+ * VL_STEP_ELEMENT_MAJOR -- f, f1, f2 are in the element-major layout (16-byte
+ * aligned).
  * For VL_F32/VL_BF16/VL_F16 storage, f holds the deviation from the
  * x-independent background `f0` (nu+2)*n*(nv+2)*n FP64 table set with
  * vl_set_background; the moment, Poisson solve and energy stay FP64. */
-enum { VL_STEP_RHO_VALID = 1 };
+enum { VL_STEP_RHO_VALID = 1, VL_STEP_ELEMENT_MAJOR = 2 };
 int vl_rk3_step(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi,
                 double dt, double *energy_out, void *workspace, int flags, void *stream);
 

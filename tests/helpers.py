@@ -32,3 +32,17 @@ def make_pair(n, nx, nu, nv, seed=0, k0=0.1, b_field=0.1, ring_j=6, dt=0.01, **k
 def random_state(shape, seed):
     rng = np.random.default_rng(seed)
     return rng.standard_normal(shape)
+
+
+def ulp_distance(a, b):
+    """Distance in units in the last place between two arrays of the same
+    IEEE storage type (torch tensors of float32 / bfloat16 / float16):
+    the bit patterns are mapped to a monotonic integer order."""
+    import torch
+    ity = {torch.float32: torch.int32, torch.bfloat16: torch.int16, torch.float16: torch.int16}[a.dtype]
+    bits = 31 if ity == torch.int32 else 15
+
+    def key(t):
+        i = t.contiguous().view(ity).cpu().numpy().astype(np.int64)
+        return np.where(i >= 0, i, -(i & ((1 << bits) - 1)))
+    return np.abs(key(a) - key(b))

@@ -98,3 +98,18 @@ def test_library_loads_and_rejects_bad_geometry():
     assert lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert lib.vl_rhs(None, None, None, None, None) == -4
     assert lib.vl_geom_create(None, ctypes.byref(h)) == -2
+
+
+def test_ulp_distance_helper():
+    """The ulp metric of the layout tests: neighbours are 1 apart, across
+    zero too, and +0 / -0 coincide."""
+    import torch
+    from helpers import ulp_distance
+    for dt, it in ((torch.float16, torch.int16), (torch.bfloat16, torch.int16), (torch.float32, torch.int32)):
+        a = torch.tensor([1.0, 3.5, -2.0, 0.0], dtype=dt)
+        up = (a.view(it) + 1).view(dt)             # away from zero for both signs
+        assert list(ulp_distance(a, up)) == [1, 1, 1, 1]
+        tiny = torch.tensor([0.0], dtype=dt).view(it) + 1
+        pos, neg = tiny.view(dt), (-tiny.view(dt))
+        assert ulp_distance(pos, neg)[0] == 2
+        assert ulp_distance(torch.tensor([0.0], dtype=dt), torch.tensor([-0.0], dtype=dt))[0] == 0

@@ -19,9 +19,12 @@ EXPORTS = (
     "vl_geom_create", "vl_geom_destroy", "vl_workspace_bytes", "vl_state_elems",
     "vl_rhs", "vl_moment0", "vl_poisson", "vl_field_energy", "vl_max_abs_acc_e",
     "vl_swap_ghosts", "vl_rk3_step", "vl_set_background", "vl_set_physics", "vl_set_stage_events",
-    "vl_workspace_init", "vl_set_tuning", "vl_get_tuning", "vl_last_error_string",
+    "vl_workspace_init", "vl_set_tuning", "vl_get_tuning", "vl_state_elems_layout",
+    "vl_convert_layout", "vl_last_error_string",
 )
 VL_TUNE_XSEG = 1
+VL_STEP_ELEMENT_MAJOR = 2
+VL_LAYOUT_REFERENCE, VL_LAYOUT_ELEMENT_MAJOR = 0, 1
 
 
 class VlGeomDesc(ctypes.Structure):
@@ -73,6 +76,8 @@ def load(path=LIB_PATH):
         "vl_workspace_init": [vp, vp, vp],
         "vl_set_tuning": [vp, i32, i32],
         "vl_get_tuning": [vp, i32, ctypes.POINTER(i32)],
+        "vl_state_elems_layout": [vp, i32, ctypes.POINTER(sz)],
+        "vl_convert_layout": [vp, vp, i32, vp, i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -168,6 +173,15 @@ class Geometry:
 
     def set_tuning(self, key, value):
         check(load().vl_set_tuning(self.handle, int(key), int(value)), "vl_set_tuning")
+
+    def state_elems_layout(self, layout):
+        e = ctypes.c_size_t()
+        check(load().vl_state_elems_layout(self.handle, int(layout), ctypes.byref(e)), "vl_state_elems_layout")
+        return int(e.value)
+
+    def convert_layout(self, src, src_layout, dst, dst_layout):
+        check(load().vl_convert_layout(self.handle, ptr(src), int(src_layout), ptr(dst), int(dst_layout),
+                                       stream_ptr()), "vl_convert_layout")
 
     def get_tuning(self, key):
         v = ctypes.c_int()

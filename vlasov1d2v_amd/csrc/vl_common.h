@@ -35,6 +35,31 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int64
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 typedef unsigned int vl_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int vl_u32x4 __attribute__((ext_vector_type(4)));
+
+// Component k (a compile-time constant after unrolling) of a 16-byte vector.
+// Named components, not v[k]: with the subscript form hipcc has read lane 0
+// for every k of an FP32 vector returned by a buffer load.
+__device__ __forceinline__ unsigned int lane(const vl_u32x4 &v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ void set_lane(vl_u32x4 &v, int k, unsigned int b) {
+    if (k == 0) v.x = b; else if (k == 1) v.y = b; else if (k == 2) v.z = b; else v.w = b;
+}
+
+// Layouts of a state array.  kRefLayout is the reference's C-order
+// (NX+2, n, NU+2, n, NV+2, n).  kElemLayout ("element-major") keeps the n^3
+// nodes of each element contiguous, (ix, iu, iv) inside, elements ordered
+// (ex, eu, ev) and padded to a pitch of EP = round_up(n^3, 8) values so that
+// every element starts 16-byte aligned for all storage types.
+enum { kRefLayout = 0, kElemLayout = 1 };
+__host__ __device__ constexpr int elem_pitch(int n) { return ((n * n * n + 7) / 8) * 8; }
+__device__ __forceinline__ vl_u32x4 buf_ld16(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, int off, vl_u32x4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+}
 
 // Storage traits of the distribution function: element type E in HBM,
 // compute type C, whether the stored value is the deviation g = (f - f0) *
@@ -53,6 +78,14 @@ template <> struct Store<0> {          // FP64: f itself
     __device__ static C ld(const E *p, int64_t i) { return p[i]; }
     __device__ static void st(E *p, int64_t i, C v) { p[i] = v; }
     __device__ static C round(C v) { return v; }
+    static constexpr int VW = 2;       // values per 16-byte vector
+    __device__ static C unpack(const vl_u32x4 &v, int k) {
+        return __builtin_bit_cast(double, k == 0 ? vl_u32x2{v.x, v.y} : vl_u32x2{v.z, v.w});
+    }
+    __device__ static void pack(vl_u32x4 &v, int k, C x) {
+        const vl_u32x2 b = __builtin_bit_cast(vl_u32x2, x);
+        if (k == 0) { v.x = b.x; v.y = b.y; } else { v.z = b.x; v.w = b.y; }
+    }
 };
 template <> struct Store<1> {          // FP32 deviation
     using E = float;
@@ -67,6 +100,9 @@ template <> struct Store<1> {          // FP32 deviation
     __device__ static C ld(const E *p, int64_t i) { return p[i]; }
     __device__ static void st(E *p, int64_t i, C v) { p[i] = v; }
     __device__ static C round(C v) { return v; }
+    static constexpr int VW = 4;
+    __device__ static C unpack(const vl_u32x4 &v, int k) { return __builtin_bit_cast(float, lane(v, k)); }
+    __device__ static void pack(vl_u32x4 &v, int k, C x) { set_lane(v, k, __builtin_bit_cast(unsigned int, x)); }
 };
 template <> struct Store<2> {          // BF16 deviation (round to nearest even)
     using E = uint16_t;
@@ -81,6 +117,15 @@ template <> struct Store<2> {          // BF16 deviation (round to nearest even)
     }
     __device__ static void st(E *p, int64_t i, C v) { p[i] = __builtin_bit_cast(uint16_t, (__bf16)v); }
     __device__ static C round(C v) { return (float)(__bf16)v; }
+    static constexpr int VW = 8;
+    __device__ static C unpack(const vl_u32x4 &v, int k) {
+        return (float)__builtin_bit_cast(__bf16, (uint16_t)(lane(v, k >> 1) >> ((k & 1) * 16)));
+    }
+    __device__ static void pack(vl_u32x4 &v, int k, C x) {
+        const unsigned int b = __builtin_bit_cast(uint16_t, (__bf16)x);
+        const unsigned int w = lane(v, k >> 1);
+        set_lane(v, k >> 1, (k & 1) ? ((w & 0xffffu) | (b << 16)) : ((w & 0xffff0000u) | b));
+    }
 };
 template <> struct Store<3> {          // FP16 deviation, scaled (round to nearest even)
     using E = uint16_t;
@@ -95,6 +140,15 @@ template <> struct Store<3> {          // FP16 deviation, scaled (round to neare
     }
     __device__ static void st(E *p, int64_t i, C v) { p[i] = __builtin_bit_cast(uint16_t, (_Float16)v); }
     __device__ static C round(C v) { return (float)(_Float16)v; }
+    static constexpr int VW = 8;
+    __device__ static C unpack(const vl_u32x4 &v, int k) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(lane(v, k >> 1) >> ((k & 1) * 16)));
+    }
+    __device__ static void pack(vl_u32x4 &v, int k, C x) {
+        const unsigned int b = __builtin_bit_cast(uint16_t, (_Float16)x);
+        const unsigned int w = lane(v, k >> 1);
+        set_lane(v, k >> 1, (k & 1) ? ((w & 0xffffu) | (b << 16)) : ((w & 0xffff0000u) | b));
+    }
 };
 
 // The tables as seen through the constant address space (scalar loads).

@@ -140,7 +140,7 @@ __device__ __forceinline__ Pencil decode_pencil(const StageArgs &s) {
     return c;
 }
 
-template <int N, int MODE, int S>
+template <int N, int MODE, int S, int L>
 __global__ __launch_bounds__(kBlock, 3) void k_stage(
     const typename Store<S>::E *__restrict__ fin,  // stage input (interior; ghosts too when !periodic_x)
     const typename Store<S>::E *__restrict__ fgh,  // array whose u/v ghost planes are used
@@ -269,12 +269,93 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
             }
         }
     }
+    // element-major layout: own elements as 16-byte vectors, v-halo values
+    constexpr int EP = elem_pitch(N);
+    constexpr int VW = ST::VW;
+    constexpr int NVS = (TV * EP) / VW;
+    constexpr int IT_V = (NVS + kBlock - 1) / kBlock;
+    constexpr int NVH = 2 * N * N;
+    constexpr int IT_VH = (NVH + kBlock - 1) / kBlock;
+    const int vrow = s.nv + 2;                       // elements per (ex, eu) row
+    int o_v[L == kElemLayout ? IT_V : 1], o_vh[L == kElemLayout ? IT_VH : 1];
+    unsigned gh_vh = 0u;
+    vl_u32x4 rv[L == kElemLayout ? IT_V : 1];
+    C rvh[L == kElemLayout ? IT_VH : 1];
+    if constexpr (L == kElemLayout) {
+        const int elem0 = (eu * vrow + v0) * EP;
+        const int nvs_valid = nvalid * EP / VW;
+#pragma unroll
+        for (int it = 0; it < IT_V; ++it) {
+            const int slot = tid + it * kBlock;
+            o_v[it] = slot < nvs_valid ? (elem0 + slot * VW) * ES : kOob;
+        }
+#pragma unroll
+        for (int it = 0; it < IT_VH; ++it) {
+            const int idx = tid + it * kBlock;
+            o_vh[it] = kOob;
+            if (idx < NVH) {
+                const int side = idx / (N * N);
+                const int r = idx - side * N * N;
+                const int ev = side ? v0 + nvalid : v0 - 1;
+                o_vh[it] = ((eu * vrow + ev) * EP + r * N + (side ? 0 : N - 1)) * ES;
+                if (ev == 0 || ev == s.nv + 1) gh_vh |= 1u << it;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT_H; ++it) {
+            const int idx = tid + it * kBlock;
+            o_h[it] = kOob;
+            gh_h &= ~(1u << it);
+            if (idx < NH) {
+                const int side = idx / (N * P);
+                const int rem = idx - side * N * P;
+                const int ix = rem / P;
+                const int q = rem - ix * P;
+                if (q < pvalid) {
+                    const int el = q / N;
+                    const int iv = q - el * N;
+                    const int eun = side ? eu + 1 : eu - 1;
+                    const int iun = side ? 0 : N - 1;
+                    o_h[it] = ((eun * vrow + v0 + el) * EP + (ix * N + iun) * N + iv) * ES;
+                    if (eun == 0 || eun == s.nu + 1) gh_h |= 1u << it;
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT_X; ++it) {
+            const int idx = tid + it * kBlock;
+            o_x[it] = kOob;
+            if (idx < NX) {
+                const int iu = idx / P;
+                const int q = idx - iu * P;
+                if (q < pvalid) {
+                    const int el = q / N;
+                    const int iv = q - el * N;
+                    o_x[it] = ((eu * vrow + v0 + el) * EP + iu * N + iv) * ES;
+                }
+            }
+        }
+    }
+    const int xplane_bytes = (L == kElemLayout ? N * N : sx) * ES;   // ix -> ix+1
     const int64_t xbytes = s.sxe * ES;              // one x element of the state
     auto rs = [&](const T *base, int e) { return make_rsrc(base + (int64_t)e * s.sxe, xbytes); };
 
     // tile rows of element e (+ v halo nodes)
     auto load_tile = [&](int e) {
         const __amdgpu_buffer_rsrc_t rin = rs(fin, e);
+        if constexpr (L == kElemLayout) {
+#pragma unroll
+            for (int it = 0; it < IT_V; ++it) rv[it] = buf_ld16(rin, o_v[it]);
+#pragma unroll
+            for (int it = 0; it < IT_VH; ++it) rvh[it] = ST::bld(rin, o_vh[it]);
+            if (gh_vh) {
+                const __amdgpu_buffer_rsrc_t rgh = rs(fgh, e);
+#pragma unroll
+                for (int it = 0; it < IT_VH; ++it)
+                    if (gh_vh & (1u << it)) rvh[it] = ST::bld(rgh, o_vh[it]);
+            }
+            return;
+        }
 #pragma unroll
         for (int it = 0; it < IT_T; ++it) rt[it] = ST::bld(rin, o_t[it]);
         if (gh_t) {
@@ -299,15 +380,47 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
     // x-plane ixn of element e (interior v nodes of the tile)
     auto load_xplane = [&](int e, int ixn) {
         const __amdgpu_buffer_rsrc_t rin = rs(fin, e);
-        const int add = ixn * sx * ES;
+        const int add = ixn * xplane_bytes;
 #pragma unroll
         for (int it = 0; it < IT_X; ++it) rr[it] = ST::bld(rin, o_x[it] == kOob ? kOob : o_x[it] + add);
     };
     auto store_tile = [&]() {
+        if constexpr (L == kElemLayout) {
+            // scatter the contiguous element runs into the (ix,iu) x q rows
 #pragma unroll
-        for (int it = 0; it < IT_T; ++it) {
-            const int idx = tid + it * kBlock;
-            if (idx < NT) ft[idx] = rt[it];
+            for (int it = 0; it < IT_V; ++it) {
+                const int slot = tid + it * kBlock;
+                if (slot < NVS) {
+#pragma unroll
+                    for (int k = 0; k < VW; ++k) {
+                        const int g = slot * VW + k;
+                        const int el = g / EP;
+                        const int rem = g - el * EP;
+                        // elements past nvalid are absent: their slot el*N + 1
+                        // would overwrite the right v-halo node at pvalid + 1
+                        if (rem < N * N * N && el < nvalid) {
+                            const int r = rem / N;            // ix*N + iu
+                            const int iv = rem - r * N;
+                            ft[r * PP + el * N + iv + 1] = ST::unpack(rv[it], k);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < IT_VH; ++it) {
+                const int idx = tid + it * kBlock;
+                if (idx < NVH) {
+                    const int side = idx / (N * N);
+                    const int r = idx - side * N * N;
+                    ft[r * PP + (side ? pvalid + 1 : 0)] = rvh[it];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < IT_T; ++it) {
+                const int idx = tid + it * kBlock;
+                if (idx < NT) ft[idx] = rt[it];
+            }
         }
 #pragma unroll
         for (int it = 0; it < IT_H; ++it) {
@@ -358,10 +471,16 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
             load_xplane(xmap(ex + 2), 0);
             if (lx_on) e_next = E[(ex + 1) * N + lx_iu];
         }
-        C rn[IT_O];
+        C rn[L == kElemLayout ? 1 : IT_O];
+        vl_u32x4 rnv[L == kElemLayout ? IT_V : 1];
         if (MODE == kModeStage) {
+            if constexpr (L == kElemLayout) {
 #pragma unroll
-            for (int it = 0; it < IT_O; ++it) rn[it] = ST::bld(rfn, o_o[it]);
+                for (int it = 0; it < IT_V; ++it) rnv[it] = buf_ld16(rfn, o_v[it]);
+            } else {
+#pragma unroll
+                for (int it = 0; it < IT_O; ++it) rn[it] = ST::bld(rfn, o_o[it]);
+            }
         }
 
         // ---- x-lines: (iu, q), line over ix ------------------------------
@@ -424,24 +543,58 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
         // ---- epilogue: combine and write rows (contiguous along v) -------
         // fn and fout may be the same array: each element is read and then
         // written by the same thread.
+        if constexpr (L == kElemLayout) {
+#pragma unroll
+            for (int it = 0; it < IT_V; ++it) {
+                const int slot = tid + it * kBlock;
+                if (slot < NVS) {
+                    vl_u32x4 w = {0u, 0u, 0u, 0u};
+#pragma unroll
+                    for (int k = 0; k < VW; ++k) {
+                        const int g = slot * VW + k;
+                        const int el = g / EP;
+                        const int rem = g - el * EP;
+                        if (rem < N * N * N) {
+                            const int r = rem / N;
+                            const int iv = rem - r * N;
+                            const int q = el * N + iv;
+                            const C Lr = acc_l[r * P + q];
+                            C val;
+                            if (MODE == kModeRhs) {
+                                val = Lr;
+                            } else if (MODE == kModeStage1) {
+                                val = ft[r * PP + q + 1] + (C)s.c2dt * Lr;
+                            } else {
+                                val = ((C)s.c0 * ST::unpack(rnv[it], k) + (C)s.c1 * ft[r * PP + q + 1]) +
+                                      (C)s.c2dt * Lr;
+                            }
+                            ST::pack(w, k, val);
+                            acc_l[r * P + q] = ST::round(val);
+                        }
+                    }
+                    buf_st16(rout, o_v[it], w);     // slots past the valid elements are dropped
+                }
+            }
+        } else {
 #pragma unroll
         for (int it = 0; it < IT_O; ++it) {
             const int idx = tid + it * kBlock;
             if (idx < NO) {
                 const int r = idx / P;
                 const int q = idx - r * P;
-                const C L = acc_l[idx];
+                const C Lr = acc_l[idx];
                 C val;
                 if (MODE == kModeRhs) {
-                    val = L;
+                    val = Lr;
                 } else if (MODE == kModeStage1) {
-                    val = ft[r * PP + q + 1] + (C)s.c2dt * L;
+                    val = ft[r * PP + q + 1] + (C)s.c2dt * Lr;
                 } else {
-                    val = ((C)s.c0 * rn[it] + (C)s.c1 * ft[r * PP + q + 1]) + (C)s.c2dt * L;
+                    val = ((C)s.c0 * rn[it] + (C)s.c1 * ft[r * PP + q + 1]) + (C)s.c2dt * Lr;
                 }
                 ST::bst(rout, o_o[it], val);      // empty slots (q >= pvalid) are dropped
                 acc_l[idx] = ST::round(val);     // the moment sees the value as stored
             }
+        }
         }
         __syncthreads();
 
@@ -506,14 +659,14 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
 // Moment partials of a state array (no RHS), one workgroup per tile
 // (ex, eu, v-chunk) in the partial layout the stage kernel writes.
 // main/grid.py:303-321.  Reduced forms: partials of g / scale.
-template <int N, int S>
+template <int N, int S, int L>
 __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store<S>::E *__restrict__ f,
                                                             StageArgs s, OpTables tab) {
     using ST = Store<S>;
     using TS = TileShape<N>;
     constexpr int TV = TS::TV, P = TS::P;
-    __shared__ double red_hi[N * P];
-    __shared__ double red_lo[N * P];
+    constexpr int EP = elem_pitch(N);
+    __shared__ double red[N * P];
     const int tid = threadIdx.x;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile % s.nchunk;
@@ -527,30 +680,55 @@ __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store
     if (tid < N * P) {
         const int ix = tid / P;
         const int q = tid - ix * P;
-        DD acc;
+        double acc = 0.0;
         if (q < pvalid) {
             const int iv = q - (q / N) * N;
-            const int64_t base = (int64_t)ex * s.sxe + ix * s.sx + (int64_t)eu * s.sue + gv0 + q;
+            const int el = q / N;
 #pragma unroll
-            for (int iu = 0; iu < N; ++iu)
-                acc.add(mul_rn((double)ST::ld(f, base + iu * s.su), tab.wq[iu * N + iv]));
+            for (int iu = 0; iu < N; ++iu) {
+                const int64_t off = L == kElemLayout
+                    ? (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + v0 + el) * EP + (ix * N + iu) * N + iv
+                    : (int64_t)ex * s.sxe + ix * s.sx + (int64_t)eu * s.sue + iu * s.su + gv0 + q;
+                acc += (double)ST::ld(f, off) * tab.wq[iu * N + iv];
+            }
         }
-        red_hi[tid] = acc.hi;
-        red_lo[tid] = acc.lo;
+        red[tid] = acc;
     }
     __syncthreads();
     if (tid < N) {
-        DD acc;
-        for (int q = 0; q < P; ++q) {
-            DD t;
-            t.hi = red_hi[tid * P + q];
-            t.lo = red_lo[tid * P + q];
-            acc.add(t);
-        }
+        double acc = 0.0;
+        for (int q = 0; q < P; ++q) acc += red[tid * P + q];
         const int tiles_per_ex = s.nu * s.nchunk;
         const int64_t slot = ((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * N + tid;
-        s.partials[slot] = ST::kDelta ? make_double2(acc.hi * s.inv_scale, acc.lo * s.inv_scale)
-                                      : make_double2(acc.hi, acc.lo);
+        s.partials[slot] = make_double2(ST::kDelta ? acc * s.inv_scale : acc, 0.0);
+    }
+}
+
+// Layout conversion of a state array (element width W bytes via type U):
+// to_elem = 1: reference -> element-major, 0: element-major -> reference.
+// Padding values of the element-major array are not written.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__ src, U *__restrict__ dst,
+                                                           int to_elem, int n, int nu, int nv, int64_t total) {
+    const int64_t ep = elem_pitch(n);
+    const int64_t sxe_e = (int64_t)(nu + 2) * (nv + 2) * ep;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+        int64_t t = i;
+        const int iv = (int)(t % n);
+        t /= n;
+        const int ev = (int)(t % (nv + 2));
+        t /= (nv + 2);
+        const int iu = (int)(t % n);
+        t /= n;
+        const int eu = (int)(t % (nu + 2));
+        t /= (nu + 2);
+        const int ix = (int)(t % n);
+        const int64_t ex = t / n;
+        const int64_t e = ex * sxe_e + ((int64_t)eu * (nv + 2) + ev) * ep + (ix * n + iu) * n + iv;
+        if (to_elem)
+            dst[e] = src[i];
+        else
+            dst[i] = src[e];
     }
 }
 

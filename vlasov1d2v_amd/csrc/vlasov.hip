@@ -26,6 +26,7 @@ struct vl_geom {
     OpTables tab;
     int TV, nchunk, tiles_per_ex, tiles;
     int xseg, pencils;     // stage kernel: x elements per pencil segment, pencils launched
+    int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
     int device;
@@ -83,7 +84,7 @@ size_t storage_bytes(int storage) {
     }
 }
 
-StageArgs make_stage_args(const vl_geom *h) {
+StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     StageArgs s;
     std::memset(&s, 0, sizeof(s));
     s.nx = h->nx;
@@ -97,7 +98,7 @@ StageArgs make_stage_args(const vl_geom *h) {
     s.acc = h->acc;
     s.bfield = h->bfield;
     s.sx = h->sx;
-    s.sxe = h->sxe;
+    s.sxe = layout == kElemLayout ? h->sxe_elem : h->sxe;
     s.su = h->su;
     s.sue = h->sue;
     s.bg = h->d_bg;
@@ -121,83 +122,87 @@ FieldArgs make_field_args(const vl_geom *h, int from_rho) {
 
 // ---- launch helpers (template dispatch over n) --------------------------
 
-template <int N, int MODE, int S>
+template <int N, int MODE, int S, int L>
 hipError_t launch_stage_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                            const double *E, const StageArgs &s, hipStream_t st) {
     using T = typename Store<S>::E;
-    hipLaunchKernelGGL((k_stage<N, MODE, S>), dim3(h->pencils), dim3(kBlock), 0, st, (const T *)fin,
+    hipLaunchKernelGGL((k_stage<N, MODE, S, L>), dim3(h->pencils), dim3(kBlock), 0, st, (const T *)fin,
                        (const T *)fgh, (const T *)fn, (T *)fout, E, h->d_unodes, h->d_vnodes, s, h->d_tab);
     return hipGetLastError();
 }
 
-template <int MODE, int S>
-hipError_t launch_stage_s(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
-                          const double *E, const StageArgs &s, hipStream_t st) {
-    switch (h->n) {
-        case 2: return launch_stage_ns<2, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 3: return launch_stage_ns<3, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 4: return launch_stage_ns<4, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 5: return launch_stage_ns<5, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 6: return launch_stage_ns<6, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 7: return launch_stage_ns<7, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 8: return launch_stage_ns<8, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 9: return launch_stage_ns<9, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        case 10: return launch_stage_ns<10, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
-        default: return hipErrorInvalidValue;
+// switch over the node count; EXPR may use the compile-time constant NN
+#define VL_DISPATCH_N(EXPR)                                  \
+    switch (h->n) {                                          \
+        case 2: { constexpr int NN = 2; return EXPR; }       \
+        case 3: { constexpr int NN = 3; return EXPR; }       \
+        case 4: { constexpr int NN = 4; return EXPR; }       \
+        case 5: { constexpr int NN = 5; return EXPR; }       \
+        case 6: { constexpr int NN = 6; return EXPR; }       \
+        case 7: { constexpr int NN = 7; return EXPR; }       \
+        case 8: { constexpr int NN = 8; return EXPR; }       \
+        case 9: { constexpr int NN = 9; return EXPR; }       \
+        case 10: { constexpr int NN = 10; return EXPR; }     \
+        default: return hipErrorInvalidValue;                \
     }
+
+template <int MODE, int S, int L>
+hipError_t launch_stage_sl(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                           const double *E, const StageArgs &s, hipStream_t st) {
+    VL_DISPATCH_N((launch_stage_ns<NN, MODE, S, L>(h, fin, fgh, fn, fout, E, s, st)))
 }
 
-// RHS mode exists for FP64 only (the drop-in DGFlux.semi_discrete_rhs)
+// RHS mode exists for FP64 in the reference layout only (the drop-in
+// DGFlux.semi_discrete_rhs); the stage modes for every storage and layout.
 template <int MODE>
-hipError_t launch_stage(const vl_geom *h, int storage, const void *fin, const void *fgh, const void *fn,
-                        void *fout, const double *E, const StageArgs &s, hipStream_t st) {
+hipError_t launch_stage(const vl_geom *h, int storage, int layout, const void *fin, const void *fgh,
+                        const void *fn, void *fout, const double *E, const StageArgs &s, hipStream_t st) {
+#define VL_STAGE_S(SS)                                                                                  \
+    return layout == kElemLayout ? launch_stage_sl<MODE, SS, kElemLayout>(h, fin, fgh, fn, fout, E, s, st) \
+                                 : launch_stage_sl<MODE, SS, kRefLayout>(h, fin, fgh, fn, fout, E, s, st)
     switch (storage) {
-        case VL_F64: return launch_stage_s<MODE, VL_F64>(h, fin, fgh, fn, fout, E, s, st);
-        case VL_F32: return launch_stage_s<MODE, VL_F32>(h, fin, fgh, fn, fout, E, s, st);
-        case VL_BF16: return launch_stage_s<MODE, VL_BF16>(h, fin, fgh, fn, fout, E, s, st);
-        case VL_F16: return launch_stage_s<MODE, VL_F16>(h, fin, fgh, fn, fout, E, s, st);
+        case VL_F64: VL_STAGE_S(VL_F64);
+        case VL_F32: VL_STAGE_S(VL_F32);
+        case VL_BF16: VL_STAGE_S(VL_BF16);
+        case VL_F16: VL_STAGE_S(VL_F16);
         default: return hipErrorInvalidValue;
     }
+#undef VL_STAGE_S
 }
 template <>
-hipError_t launch_stage<kModeRhs>(const vl_geom *h, int storage, const void *fin, const void *fgh,
+hipError_t launch_stage<kModeRhs>(const vl_geom *h, int storage, int layout, const void *fin, const void *fgh,
                                   const void *fn, void *fout, const double *E, const StageArgs &s,
                                   hipStream_t st) {
-    if (storage != VL_F64) return hipErrorInvalidValue;
-    return launch_stage_s<kModeRhs, VL_F64>(h, fin, fgh, fn, fout, E, s, st);
+    if (storage != VL_F64 || layout != kRefLayout) return hipErrorInvalidValue;
+    return launch_stage_sl<kModeRhs, VL_F64, kRefLayout>(h, fin, fgh, fn, fout, E, s, st);
 }
 
-template <int N, int S>
-hipError_t launch_moment_ns(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
+template <int N, int S, int L>
+hipError_t launch_moment_nsl(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
     using T = typename Store<S>::E;
-    hipLaunchKernelGGL((k_moment_partials<N, S>), dim3(h->tiles), dim3(kBlock), 0, st, (const T *)f, s, h->tab);
+    hipLaunchKernelGGL((k_moment_partials<N, S, L>), dim3(h->tiles), dim3(kBlock), 0, st, (const T *)f, s,
+                       h->tab);
     return hipGetLastError();
 }
 
-template <int S>
-hipError_t launch_moment_s(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
-    switch (h->n) {
-        case 2: return launch_moment_ns<2, S>(h, f, s, st);
-        case 3: return launch_moment_ns<3, S>(h, f, s, st);
-        case 4: return launch_moment_ns<4, S>(h, f, s, st);
-        case 5: return launch_moment_ns<5, S>(h, f, s, st);
-        case 6: return launch_moment_ns<6, S>(h, f, s, st);
-        case 7: return launch_moment_ns<7, S>(h, f, s, st);
-        case 8: return launch_moment_ns<8, S>(h, f, s, st);
-        case 9: return launch_moment_ns<9, S>(h, f, s, st);
-        case 10: return launch_moment_ns<10, S>(h, f, s, st);
-        default: return hipErrorInvalidValue;
-    }
+template <int S, int L>
+hipError_t launch_moment_sl(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
+    VL_DISPATCH_N((launch_moment_nsl<NN, S, L>(h, f, s, st)))
 }
 
-hipError_t launch_moment(const vl_geom *h, int storage, const void *f, const StageArgs &s, hipStream_t st) {
+hipError_t launch_moment(const vl_geom *h, int storage, int layout, const void *f, const StageArgs &s,
+                         hipStream_t st) {
+#define VL_MOM_S(SS)                                                                   \
+    return layout == kElemLayout ? launch_moment_sl<SS, kElemLayout>(h, f, s, st) \
+                                 : launch_moment_sl<SS, kRefLayout>(h, f, s, st)
     switch (storage) {
-        case VL_F64: return launch_moment_s<VL_F64>(h, f, s, st);
-        case VL_F32: return launch_moment_s<VL_F32>(h, f, s, st);
-        case VL_BF16: return launch_moment_s<VL_BF16>(h, f, s, st);
-        case VL_F16: return launch_moment_s<VL_F16>(h, f, s, st);
+        case VL_F64: VL_MOM_S(VL_F64);
+        case VL_F32: VL_MOM_S(VL_F32);
+        case VL_BF16: VL_MOM_S(VL_BF16);
+        case VL_F16: VL_MOM_S(VL_F16);
         default: return hipErrorInvalidValue;
     }
+#undef VL_MOM_S
 }
 
 double2 *ws_partials(const vl_geom *h, void *ws) {
@@ -424,11 +429,12 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         return fail(VL_E_ARG, "grid too large");
     }
     h->tiles = (int)tiles;
-    // pencils of the stage kernel: enough x segments for ~4 waves of 3
-    // workgroups on each of 256 CUs (host-tunable with vl_set_tuning)
+    // pencils of the stage kernel: about 13000 workgroups (measured best on
+    // MI355X for the 116.6M-DOF grid: 4 x elements per pencil); host-tunable
+    // with vl_set_tuning
     {
         const int base = d->nu * h->nchunk;
-        int nseg = (3072 + base - 1) / base;
+        int nseg = (13000 + base - 1) / base;
         nseg = std::max(1, std::min(nseg, d->nx));
         h->xseg = (d->nx + nseg - 1) / nseg;
         h->pencils = base * ((d->nx + h->xseg - 1) / h->xseg);
@@ -438,8 +444,9 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     h->sx = h->sue * (d->nu + 2);
     h->sxe = h->sx * n;
     h->state_elems = (size_t)h->sxe * (d->nx + 2);
+    h->sxe_elem = (int64_t)(d->nu + 2) * (d->nv + 2) * elem_pitch(n);
     // buffer addressing: one x element of the state must span < 2^31 bytes
-    if (h->sxe * (int64_t)storage_bytes(d->storage) >= 2147483647LL) {
+    if (h->sxe_elem * (int64_t)storage_bytes(d->storage) >= 2147483647LL) {
         delete h;
         return fail(VL_E_ARG, "one x element of the state exceeds 2^31 bytes");
     }
@@ -522,7 +529,7 @@ int vl_rhs(vl_geom_t h, const double *f, const double *E, double *out, void *str
     StageArgs s = make_stage_args(h);
     s.periodic_x = 0;
     s.write_partials = 0;
-    VL_HIP(launch_stage<kModeRhs>(h, VL_F64, f, f, nullptr, out, E, s, (hipStream_t)stream));
+    VL_HIP(launch_stage<kModeRhs>(h, VL_F64, kRefLayout, f, f, nullptr, out, E, s, (hipStream_t)stream));
     return VL_OK;
 }
 
@@ -533,7 +540,7 @@ int vl_moment0(vl_geom_t h, const double *f, double *rho, void *ws, void *stream
     hipStream_t st = (hipStream_t)stream;
     StageArgs s = make_stage_args(h);
     s.partials = ws_partials(h, ws);
-    VL_HIP(launch_moment(h, VL_F64, f, s, st));
+    VL_HIP(launch_moment(h, VL_F64, kRefLayout, f, s, st));
     VL_HIP(launch_reduce(h, ws, rho, st));
     return VL_OK;
 }
@@ -565,12 +572,9 @@ int vl_max_abs_acc_e(vl_geom_t h, const double *E, double *out, void *stream) {
     return VL_OK;
 }
 
-int vl_swap_ghosts(vl_geom_t h, void *f, void *stream) {
-    if (int r = check_handle(h)) return r;
-    if (!f) return fail(VL_E_NULL, "vl_swap_ghosts: null pointer");
+int swap_ghosts_layout(vl_geom_t h, void *f, int layout, void *stream) {
     const size_t eb = storage_bytes(h->storage);
-    if (!aligned(f, eb)) return fail(VL_E_ALIGN, "vl_swap_ghosts: misaligned");
-    const int64_t plane = h->sxe;
+    const int64_t plane = layout == kElemLayout ? h->sxe_elem : h->sxe;
     const int64_t blocks = std::min<int64_t>((2 * plane + kBlock - 1) / kBlock, 8192);
     hipStream_t st = (hipStream_t)stream;
     if (eb == 8)
@@ -582,6 +586,13 @@ int vl_swap_ghosts(vl_geom_t h, void *f, void *stream) {
                            h->nx);
     VL_HIP(hipGetLastError());
     return VL_OK;
+}
+
+int vl_swap_ghosts(vl_geom_t h, void *f, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!f) return fail(VL_E_NULL, "vl_swap_ghosts: null pointer");
+    if (!aligned(f, storage_bytes(h->storage))) return fail(VL_E_ALIGN, "vl_swap_ghosts: misaligned");
+    return swap_ghosts_layout(h, f, kRefLayout, stream);
 }
 
 int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double dt,
@@ -596,20 +607,23 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
         return fail(VL_E_ARG, "vl_rk3_step: reduced storage needs vl_set_background first");
     void *f = fv, *f1 = f1v, *f2 = f2v;
     const int S = h->storage;
+    const int LAY = (flags & VL_STEP_ELEMENT_MAJOR) ? kElemLayout : kRefLayout;
+    if (LAY == kElemLayout && (!aligned(fv, 16) || !aligned(f1v, 16) || !aligned(f2v, 16)))
+        return fail(VL_E_ALIGN, "vl_rk3_step: element-major arrays must be 16-byte aligned");
     hipStream_t st = (hipStream_t)stream;
-    if (int r = vl_swap_ghosts(h, f, stream)) return r;
-    StageArgs s = make_stage_args(h);
+    if (int r = swap_ghosts_layout(h, f, LAY, stream)) return r;
+    StageArgs s = make_stage_args(h, LAY);
     s.periodic_x = 1;
     s.write_partials = 1;
     s.partials = ws_partials(h, ws);
-    if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, f, s, st));
+    if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, LAY, f, s, st));
     // stage 1: f1 = f + dt L(f)                       (main/timestep.py:153-163)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
     s.c0 = 0.0;
     s.c1 = 1.0;
     s.c2dt = dt;
     VL_HIP(record_stage_event(h, 0, st));
-    VL_HIP(launch_stage<kModeStage1>(h, S, f, f, nullptr, f1, E, s, st));
+    VL_HIP(launch_stage<kModeStage1>(h, S, LAY, f, f, nullptr, f1, E, s, st));
     VL_HIP(record_stage_event(h, 1, st));
     // stage 2: f2 = 3/4 f + 1/4 f1 + 1/4 dt L(f1)     (main/timestep.py:165-182)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
@@ -617,7 +631,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.c1 = 1.0 / 4.0;
     s.c2dt = (1.0 / 4.0) * dt;
     VL_HIP(record_stage_event(h, 2, st));
-    VL_HIP(launch_stage<kModeStage>(h, S, f1, f, f, f2, E, s, st));
+    VL_HIP(launch_stage<kModeStage>(h, S, LAY, f1, f, f, f2, E, s, st));
     VL_HIP(record_stage_event(h, 3, st));
     // stage 3: f = 1/3 f + 2/3 f2 + 2/3 dt L(f2); E, phi, energy of f2's field
     VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st));
@@ -625,8 +639,42 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.c1 = 2.0 / 3.0;
     s.c2dt = (2.0 / 3.0) * dt;
     VL_HIP(record_stage_event(h, 4, st));
-    VL_HIP(launch_stage<kModeStage>(h, S, f2, f, f, f, E, s, st));
+    VL_HIP(launch_stage<kModeStage>(h, S, LAY, f2, f, f, f, E, s, st));
     VL_HIP(record_stage_event(h, 5, st));
+    return VL_OK;
+}
+
+int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems) {
+    if (int r = check_handle(h)) return r;
+    if (!elems) return fail(VL_E_NULL, "null elems");
+    if (layout != VL_LAYOUT_REFERENCE && layout != VL_LAYOUT_ELEMENT_MAJOR) return fail(VL_E_ARG, "bad layout");
+    *elems = layout == VL_LAYOUT_ELEMENT_MAJOR ? (size_t)h->sxe_elem * (h->nx + 2) : h->state_elems;
+    return VL_OK;
+}
+
+int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, int dst_layout, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!src || !dst) return fail(VL_E_NULL, "vl_convert_layout: null pointer");
+    if (src == dst) return fail(VL_E_ARG, "vl_convert_layout: in-place conversion not supported");
+    const bool to_elem = src_layout == VL_LAYOUT_REFERENCE && dst_layout == VL_LAYOUT_ELEMENT_MAJOR;
+    const bool from_elem = src_layout == VL_LAYOUT_ELEMENT_MAJOR && dst_layout == VL_LAYOUT_REFERENCE;
+    if (!to_elem && !from_elem) return fail(VL_E_ARG, "vl_convert_layout: unsupported layout pair");
+    const size_t eb = storage_bytes(h->storage);
+    if (!aligned(src, eb) || !aligned(dst, eb)) return fail(VL_E_ALIGN, "vl_convert_layout: misaligned");
+    const int64_t total = (int64_t)h->state_elems;
+    const int64_t blocks = std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
+    hipStream_t st = (hipStream_t)stream;
+    const int te = to_elem ? 1 : 0;
+    if (eb == 8)
+        hipLaunchKernelGGL(k_convert_layout<double>, dim3(blocks), dim3(kBlock), 0, st, (const double *)src,
+                           (double *)dst, te, h->n, h->nu, h->nv, total);
+    else if (eb == 4)
+        hipLaunchKernelGGL(k_convert_layout<float>, dim3(blocks), dim3(kBlock), 0, st, (const float *)src,
+                           (float *)dst, te, h->n, h->nu, h->nv, total);
+    else
+        hipLaunchKernelGGL(k_convert_layout<uint16_t>, dim3(blocks), dim3(kBlock), 0, st,
+                           (const uint16_t *)src, (uint16_t *)dst, te, h->n, h->nu, h->nv, total);
+    VL_HIP(hipGetLastError());
     return VL_OK;
 }
 

@@ -66,6 +66,7 @@ class Stepper:
         self._stages = None
         self._rho_token = None
         self.elapsed = None
+        self._res = None          # resident element-major state: dict or None
 
     # -- helpers -----------------------------------------------------------
     def _stage_buffers(self, arr):
@@ -87,6 +88,35 @@ class Stepper:
         self.adapt_time_step(get_max_speeds(grids, elliptic, self._acc(dg_flux, refs)),
                              grids.x.dx, grids.u.dx, grids.v.dx)
 
+    # -- resident element-major stepping ------------------------------------
+    def resident_begin(self, distribution, grids, dg_flux, elliptic):
+        """Copy distribution.arr into the element-major layout (each element's
+        n^3 nodes contiguous) and step there until resident_end(); the
+        reference-layout distribution.arr is stale in between."""
+        import torch
+        arr = distribution.arr
+        _check_state(arr, grids.shape, "resident_begin", allow_reduced=True)
+        g = dg_flux._geom(grids, elliptic, STORAGE_CODES[storage_of(arr)])
+        n_em = g.state_elems_layout(_lib.VL_LAYOUT_ELEMENT_MAJOR)
+        r = self._res
+        if r is None or r["bufs"][0].numel() != n_em or r["bufs"][0].dtype != arr.dtype \
+                or r["bufs"][0].device != arr.device:
+            bufs = tuple(torch.zeros(n_em, dtype=arr.dtype, device=arr.device) for _ in range(3))
+        else:
+            bufs = r["bufs"]
+        g.convert_layout(arr, _lib.VL_LAYOUT_REFERENCE, bufs[0], _lib.VL_LAYOUT_ELEMENT_MAJOR)
+        self._res = {"bufs": bufs, "dist": distribution, "geom": g, "token": None, "active": True}
+
+    def resident_end(self, distribution):
+        """Write the resident state back into distribution.arr (reference layout)."""
+        r = self._res
+        if r is None or not r["active"]:
+            return
+        r["geom"].convert_layout(r["bufs"][0], _lib.VL_LAYOUT_ELEMENT_MAJOR, distribution.arr,
+                                 _lib.VL_LAYOUT_REFERENCE)
+        r["active"] = False
+        self._rho_token = None
+
     # -- one step ------------------------------------------------------------
     def nonlinear_ssp_rk(self, distribution, basis, elliptic, grids, dg_flux, refs=None, energy_out=None):
         """One SSP-RK3 step (main/timestep.py:141-187), fused on the GPU.
@@ -99,7 +129,12 @@ class Stepper:
         arr = distribution.arr
         _check_state(arr, grids.shape, "nonlinear_ssp_rk", allow_reduced=True)
         g = dg_flux._geom(grids, elliptic, STORAGE_CODES[storage_of(arr)])
-        f1, f2 = self._stage_buffers(arr)
+        res = self._res if (self._res is not None and self._res["active"]
+                            and self._res["dist"] is distribution) else None
+        if res is not None:
+            arr, f1, f2 = res["bufs"]
+        else:
+            f1, f2 = self._stage_buffers(arr)
         dev = arr.device
         if elliptic.electric_field is None or elliptic.electric_field.device != dev:
             elliptic.electric_field = torch.zeros((grids.x.res_ghosts, grids.x.order), dtype=torch.float64,
@@ -109,6 +144,8 @@ class Stepper:
         ws = g.workspace(dev)
         token = (arr.data_ptr(), arr._version, ws.data_ptr())
         flags = _lib.VL_STEP_RHO_VALID if token == self._rho_token else 0
+        if res is not None:
+            flags |= _lib.VL_STEP_ELEMENT_MAJOR
         if self.dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
         _lib.check(_lib.load().vl_rk3_step(
@@ -172,12 +209,14 @@ class Stepper:
                 steps = int(np.ceil((self.final_time - self.time) / self.dt - 1e-9))
             energies = torch.empty(steps + 1, dtype=torch.float64, device=dev)
             energies[0] = e0
+            self.resident_begin(distribution, grids, dg_flux, elliptic)
             for k in range(steps):
                 self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
                                       energy_out=energies[k + 1:k + 2])
                 self.time += self.dt
                 self.steps_counter += 1
                 self.time_array = np.append(self.time_array, self.time)
+            self.resident_end(distribution)
             torch.cuda.synchronize(dev)
             self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
         self.elapsed = _time.time() - t0
