@@ -91,8 +91,13 @@ def main():
     head = forms["f64"] if "f64" in forms else forms[storages[0]]
     N = w.dof
     s_bytes = head["bytes_per_value"]
-    stage_bytes = [2 * N * s_bytes, 3 * N * s_bytes, 3 * N * s_bytes]   # algorithmic bytes per launch
-    achieved = sum(stage_bytes) / (sum(head["stage_ms"]) / 1e3) / 1e9
+    # dominant kernel: the stage-2/3 launches (k_stage MODE 2: read the stage
+    # input and f^n, write the stage output) -> 3 N s algorithmic bytes each
+    launch_bytes = 3 * N * s_bytes
+    launch_ms = 0.5 * (head["stage_ms"][1] + head["stage_ms"][2])
+    achieved = launch_bytes / (launch_ms / 1e3) / 1e9
+    kname = f"k_stage<{w.n},2,{STORAGE_CODES[head['storage']]},{1 if args.layout == 'elem' else 0}>"
+    traffic = profiled_traffic(kname, head["storage"])
     out = {
         "metric": "RK3 DOF-updates/s at 116.6M DOF, FP64 vs BF16-store; % HBM roofline",
         "value": head["dof_updates_per_s"],
@@ -110,18 +115,40 @@ def main():
                    "dt": w.dt, "storage": head["storage"], "layout": args.layout,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage (fused RHS + stage combination + moment partials), "
-                               + head["storage"],
-                     "stage_ms": head["stage_ms"], "stage_algorithmic_bytes": stage_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname + " (fused RHS + stage combination + moment partials; stages 2 and 3)",
+                     "launch_ms": launch_ms, "algorithmic_bytes_per_launch": launch_bytes,
+                     "stage_ms": head["stage_ms"]},
         "forms": {k: {kk: v[kk] for kk in ("dof_updates_per_s", "ms_per_step", "stage_ms",
-                                           "step_hbm_frac", "finite")} for k, v in forms.items()},
+                                           "stage23_hbm_frac", "step_hbm_frac", "finite")}
+                  for k, v in forms.items()},
     }
     if "bf16" in forms and "f64" in forms:
         out["bf16_over_f64"] = forms["bf16"]["dof_updates_per_s"] / forms["f64"]["dof_updates_per_s"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
+
+
+def profiled_traffic(kernel, storage):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE profile of this workload (counter readings in KiB per
+    dispatch, scaled by the copy-kernel calibration); None if not profiled."""
+    path = os.path.join(ROOT, "profiles", "k_stage_ring64_n9_64x50x50.md")
+    if not os.path.exists(path):
+        return None
+    cal, rows = None, {}
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if len(t) == 5 and t[0] == "calibration":
+                cal = (float(t[2]), float(t[4]))
+            elif len(t) == 5 and t[0] == "traffic":
+                rows[(t[1], t[2])] = (float(t[3]), float(t[4]))
+    if cal is None or (kernel, storage) not in rows:
+        return None
+    fetch, write = rows[(kernel, storage)]
+    return (fetch * cal[0] + write * cal[1]) * 1024.0
 
 
 def measure(w, storage, K, W, layout, dev):
@@ -164,6 +191,8 @@ def measure(w, storage, K, W, layout, dev):
     res = {"storage": storage, "bytes_per_value": bpv, "ms_per_step": ms / K,
            "dof_updates_per_s": w.dof * K / (ms / 1e3), "stage_ms": stage_ms,
            "step_hbm_frac": 8 * w.dof * bpv / (ms / K / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "stage23_hbm_frac": 3 * w.dof * bpv / (0.5 * (stage_ms[1] + stage_ms[2]) / 1e3) / 1e9
+                               / HBM_PEAK_GBS,
            "finite": bool(torch.isfinite(energies).all().item())}
     del run
     return res
