@@ -103,8 +103,10 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
 
 /* Launch-shape tuning of the stage kernel (no counterpart in the reference).
  * VL_TUNE_XSEG: x elements each workgroup walks (default: chosen so that the
- * launch has about 13000 workgroups). */
-enum { VL_TUNE_XSEG = 1 };
+ * launch has about 13000 workgroups).
+ * VL_TUNE_MFMA: 1 (default) runs the BF16/FP16 stages of the element-major
+ * layout on the matrix cores, 0 on the vector ALUs. */
+enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2 };
 int vl_set_tuning(vl_geom_t h, int key, int value);
 int vl_get_tuning(vl_geom_t h, int key, int *value);
 

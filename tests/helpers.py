@@ -46,3 +46,14 @@ def ulp_distance(a, b):
         i = t.contiguous().view(ity).cpu().numpy().astype(np.int64)
         return np.where(i >= 0, i, -(i & ((1 << bits) - 1)))
     return np.abs(key(a) - key(b))
+
+
+def significant_ulp_distance(a, b, floor=1e-3):
+    """ulp_distance over the entries whose magnitude is at least `floor`
+    times the array maximum.  Near zero (values that cancelled down from
+    larger intermediates) a rounding flip of an intermediate shows up as many
+    ulps of a tiny number; those entries are covered by a norm bound."""
+    import torch
+    big = torch.maximum(a.float().abs(), b.float().abs())
+    mask = (big >= floor * float(a.float().abs().max())).cpu().numpy()
+    return ulp_distance(a, b)[mask]

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import interior, make_pair, random_state, rel_l2, rel_max, ulp_distance
+from helpers import interior, make_pair, random_state, rel_l2, rel_max, significant_ulp_distance, ulp_distance
 
 pytestmark = pytest.mark.gpu
 
@@ -111,16 +111,18 @@ def test_unfused_step_matches_fused(cuda):
 @pytest.mark.parametrize("n,nx,nu,nv", [(9, 3, 4, 5), (4, 5, 3, 7), (7, 2, 6, 4)])
 def test_element_major_resident_matches_reference_layout(cuda, storage, n, nx, nu, nv):
     """Steps in the resident element-major layout give the same state as steps
-    in the reference layout: same arithmetic in another memory order.  The
-    two kernel instantiations may contract multiply-adds differently, so the
-    bar is rounding level: 1e-14 relative for FP64; for the reduced forms at
-    most 2 ulps of the storage format on any stored value (a 1-ulp FP32
-    difference flips a BF16/FP16 rounding now and then) and on fewer than
-    1% of the values."""
+    in the reference layout: same arithmetic in another memory order (both on
+    the vector-ALU stage kernel).  The two kernel instantiations may contract
+    multiply-adds differently, so the bar is rounding level: 1e-14 relative
+    for FP64; for the reduced forms at most 2 ulps of the storage format on
+    any stored value (a 1-ulp FP32 difference flips a BF16/FP16 rounding now
+    and then) and on fewer than 1% of the values."""
     import torch
-    from vlasov1d2v_amd import Run
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
     a = Run(n, nx, nu, nv, dt=0.004, storage=storage)
     b = Run(n, nx, nu, nv, dt=0.004, storage=storage)
+    b.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, 0)
     for _ in range(3):
         a.step()
     b.resident_begin()
@@ -144,3 +146,48 @@ def test_element_major_resident_matches_reference_layout(cuda, storage, n, nx, n
         assert np.mean(d > 0) < 1e-2
         tol_e = 1e-4
     assert rel_l2(b.elliptic.electric_field.cpu().numpy(), a.elliptic.electric_field.cpu().numpy()) < tol_e
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("n,nx,nu,nv", [(9, 3, 4, 5), (2, 3, 3, 70), (3, 2, 4, 30), (4, 3, 3, 17),
+                                        (5, 2, 3, 11), (6, 3, 4, 9), (7, 2, 5, 6), (8, 3, 3, 5),
+                                        (10, 2, 3, 5)])
+def test_matrix_core_stage_matches_vector_stage(cuda, storage, n, nx, nu, nv):
+    """The BF16/FP16 stage kernel on the matrix cores (operator split into
+    hi + lo 16-bit parts, about 2^-17 relative, FP32 accumulation) against
+    the vector-ALU kernel (FP32 operator) after 3 resident steps.  The
+    operators differ below the storage precision, so an intermediate stage
+    value now and then rounds the other way, and later stages can cancel
+    such a value down to a small one.  Bars: fewer than 1% of the values
+    differ; no value differs by more than 4 storage ulps of the state's
+    maximum; 2^-13 relative L2 on the whole state (far below one storage
+    ulp).  The shapes cover every n, partial v-chunks and the
+    velocity-boundary pencils."""
+    import torch
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    runs = []
+    for mfma in (0, 1):
+        r = Run(n, nx, nu, nv, dt=0.004, storage=storage)
+        r.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
+        r.resident_begin()
+        for _ in range(3):
+            r.step()
+        r.resident_end()
+        runs.append(r)
+    torch.cuda.synchronize()
+    d = ulp_distance(runs[1].dist.arr, runs[0].dist.arr)
+    g0 = runs[0].dist.arr.float().cpu().numpy()
+    g1 = runs[1].dist.arr.float().cpu().numpy()
+    err = rel_l2(g1, g0)
+    ulp = 2.0 ** -8 if storage == "bf16" else 2.0 ** -11
+    worst = float(np.max(np.abs(g1 - g0))) / float(np.max(np.abs(g0)))
+    print(f"{storage} n={n}: differing fraction {np.mean(d > 0):.2e}, max |diff| {worst / ulp:.2f} ulp of max, "
+          f"rel-L2 {err:.2e}", flush=True)
+    assert np.all(np.isfinite(g1))
+    assert np.mean(d > 0) < 1e-2
+    assert worst <= 4 * ulp
+    assert err < 2.0 ** -13
+    e0 = runs[0].elliptic.electric_field.cpu().numpy()
+    e1 = runs[1].elliptic.electric_field.cpu().numpy()
+    assert rel_l2(e1, e0) < 1e-4

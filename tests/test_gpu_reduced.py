@@ -45,3 +45,24 @@ def test_reduced_storage_initial_state_is_exact_background_plus_deviation(cuda):
     fb = b.dist.total().cpu().numpy()
     assert rel_l2(fb, fa) < 1e-9
     assert b.dist.arr.dtype == torch.float32
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_matrix_core_path_is_as_accurate_as_vector_path(cuda, storage):
+    """Against the FP64 path, the matrix-core stage kernel (the default for
+    the 16-bit forms in the resident layout) is as accurate as the
+    vector-ALU kernel: its operator rounding (hi + lo 16-bit parts) is far
+    below the storage rounding."""
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    f64, e64 = _run("f64")
+    errs = {}
+    for mfma in (0, 1):
+        run = Run(6, 8, 10, 10, seed=3, dt=0.005, storage=storage)
+        run.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
+        st = run.run(20)
+        f = run.dist.total().cpu().numpy()[1:-1, :, 1:-1, :, 1:-1, :]
+        errs[mfma] = (rel_l2(f, f64), rel_l2(st.field_energy, e64))
+    print(f"{storage}: vector {errs[0]}, matrix {errs[1]}", flush=True)
+    assert errs[1][0] <= 1.1 * errs[0][0]
+    assert errs[1][1] <= 1.1 * errs[0][1] + 1e-7

@@ -23,6 +23,7 @@ EXPORTS = (
     "vl_convert_layout", "vl_last_error_string",
 )
 VL_TUNE_XSEG = 1
+VL_TUNE_MFMA = 2
 VL_STEP_ELEMENT_MAJOR = 2
 VL_LAYOUT_REFERENCE, VL_LAYOUT_ELEMENT_MAJOR = 0, 1
 

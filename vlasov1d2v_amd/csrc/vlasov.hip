@@ -15,6 +15,7 @@
 #include "vl_common.h"
 #include "vl_field.h"
 #include "vl_stage.h"
+#include "vl_stage_mx.h"
 
 #include <vector>
 
@@ -26,6 +27,7 @@ struct vl_geom {
     OpTables tab;
     int TV, nchunk, tiles_per_ex, tiles;
     int xseg, pencils;     // stage kernel: x elements per pencil segment, pencils launched
+    int mfma;              // 1: matrix-core stage kernel for the 16-bit forms (element-major)
     int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
@@ -146,6 +148,21 @@ hipError_t launch_stage_ns(const vl_geom *h, const void *fin, const void *fgh, c
         default: return hipErrorInvalidValue;                \
     }
 
+template <int N, int MODE, int S>
+hipError_t launch_stage_mx_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                              const double *E, const StageArgs &s, hipStream_t st) {
+    hipLaunchKernelGGL((k_stage_mx<N, MODE, S>), dim3(h->pencils), dim3(kBlock), 0, st, (const uint16_t *)fin,
+                       (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
+                       h->d_vnodes, s, h->d_tab);
+    return hipGetLastError();
+}
+
+template <int MODE, int S>
+hipError_t launch_stage_mx(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                           const double *E, const StageArgs &s, hipStream_t st) {
+    VL_DISPATCH_N((launch_stage_mx_ns<NN, MODE, S>(h, fin, fgh, fn, fout, E, s, st)))
+}
+
 template <int MODE, int S, int L>
 hipError_t launch_stage_sl(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                            const double *E, const StageArgs &s, hipStream_t st) {
@@ -160,6 +177,10 @@ hipError_t launch_stage(const vl_geom *h, int storage, int layout, const void *f
 #define VL_STAGE_S(SS)                                                                                  \
     return layout == kElemLayout ? launch_stage_sl<MODE, SS, kElemLayout>(h, fin, fgh, fn, fout, E, s, st) \
                                  : launch_stage_sl<MODE, SS, kRefLayout>(h, fin, fgh, fn, fout, E, s, st)
+    if (layout == kElemLayout && h->mfma) {
+        if (storage == VL_BF16) return launch_stage_mx<MODE, VL_BF16>(h, fin, fgh, fn, fout, E, s, st);
+        if (storage == VL_F16) return launch_stage_mx<MODE, VL_F16>(h, fin, fgh, fn, fout, E, s, st);
+    }
     switch (storage) {
         case VL_F64: VL_STAGE_S(VL_F64);
         case VL_F32: VL_STAGE_S(VL_F32);
@@ -437,6 +458,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         int nseg = (13000 + base - 1) / base;
         nseg = std::max(1, std::min(nseg, d->nx));
         h->xseg = (d->nx + nseg - 1) / nseg;
+        h->mfma = 1;
         h->pencils = base * ((d->nx + h->xseg - 1) / h->xseg);
     }
     h->su = (int64_t)(d->nv + 2) * n;
@@ -695,6 +717,11 @@ int vl_set_tuning(vl_geom_t h, int key, int value) {
         h->pencils = h->nu * h->nchunk * ((h->nx + h->xseg - 1) / h->xseg);
         return VL_OK;
     }
+    if (key == VL_TUNE_MFMA) {
+        if (value != 0 && value != 1) return fail(VL_E_ARG, "mfma must be 0 or 1");
+        h->mfma = value;
+        return VL_OK;
+    }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
 }
 
@@ -703,6 +730,10 @@ int vl_get_tuning(vl_geom_t h, int key, int *value) {
     if (!value) return fail(VL_E_NULL, "vl_get_tuning: null value");
     if (key == VL_TUNE_XSEG) {
         *value = h->xseg;
+        return VL_OK;
+    }
+    if (key == VL_TUNE_MFMA) {
+        *value = h->mfma;
         return VL_OK;
     }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
