@@ -105,8 +105,11 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
  * VL_TUNE_XSEG: x elements each workgroup walks (default: chosen so that the
  * launch has about 13000 workgroups).
  * VL_TUNE_MFMA: 1 (default) runs the BF16/FP16 stages of the element-major
- * layout on the matrix cores, 0 on the vector ALUs. */
-enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2 };
+ * layout on the matrix cores, 0 on the vector ALUs.
+ * VL_TUNE_ROW_SPLIT: 1 (default) makes vl_resident_layout report the
+ * row-split layout for BF16/FP16 storage with n = 8 or 9 (while
+ * VL_TUNE_MFMA is 1); 0 keeps the element-major layout. */
+enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2, VL_TUNE_ROW_SPLIT = 3 };
 int vl_set_tuning(vl_geom_t h, int key, int value);
 int vl_get_tuning(vl_geom_t h, int key, int *value);
 
@@ -119,8 +122,18 @@ int vl_state_elems(vl_geom_t h, size_t *elems);
  * ((ix, iu, iv) inside, elements ordered (ex, eu, ev)) with a pitch of
  * round_up(n^3, 8) values per element, so that a tile of elements is one
  * contiguous run.  It is the resident layout of Stepper.main_loop. */
-enum { VL_LAYOUT_REFERENCE = 0, VL_LAYOUT_ELEMENT_MAJOR = 1 };
+/* VL_LAYOUT_ROW_SPLIT (n = 8 or 9 only): elements ordered and padded as in
+ * the element-major layout; inside an element the first eight nodes of each
+ * v-line (ix, iu) form one 16-byte row (rows in (ix, iu) order), and for
+ * n = 9 the ninth nodes of all lines follow in (ix, iu) order.  It is the
+ * resident layout of the matrix-core stage kernel of the 16-bit forms. */
+enum { VL_LAYOUT_REFERENCE = 0, VL_LAYOUT_ELEMENT_MAJOR = 1, VL_LAYOUT_ROW_SPLIT = 2 };
 int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems);
+
+/* The layout vl_rk3_step runs fastest in for this handle's storage and n:
+ * VL_LAYOUT_ROW_SPLIT or VL_LAYOUT_ELEMENT_MAJOR (no counterpart in the
+ * reference, which keeps one layout). */
+int vl_resident_layout(vl_geom_t h, int *layout);
 
 /* dst = src in the other layout (out of place; element-major padding values
  * are not written). */
@@ -173,11 +186,12 @@ int vl_swap_ghosts(vl_geom_t h, void *f, void *stream);
  * flags: VL_STEP_RHO_VALID -- the workspace already holds the moment partials
  * of f from the previous vl_rk3_step on the same f (skips one read of f).
  * VL_STEP_ELEMENT_MAJOR -- f, f1, f2 are in the element-major layout (16-byte
- * aligned).
+ * aligned).  VL_STEP_ROW_SPLIT -- f, f1, f2 are in the row-split layout
+ * (16-byte aligned; BF16/FP16 storage, n = 8 or 9).
  * For VL_F32/VL_BF16/VL_F16 storage, f holds the deviation from the
  * x-independent background `f0` (nu+2)*n*(nv+2)*n FP64 table set with
  * vl_set_background; the moment, Poisson solve and energy stay FP64. */
-enum { VL_STEP_RHO_VALID = 1, VL_STEP_ELEMENT_MAJOR = 2 };
+enum { VL_STEP_RHO_VALID = 1, VL_STEP_ELEMENT_MAJOR = 2, VL_STEP_ROW_SPLIT = 4 };
 int vl_rk3_step(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi,
                 double dt, double *energy_out, void *workspace, int flags, void *stream);
 

@@ -148,46 +148,117 @@ def test_element_major_resident_matches_reference_layout(cuda, storage, n, nx, n
     assert rel_l2(b.elliptic.electric_field.cpu().numpy(), a.elliptic.electric_field.cpu().numpy()) < tol_e
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
-@pytest.mark.parametrize("n,nx,nu,nv", [(9, 3, 4, 5), (2, 3, 3, 70), (3, 2, 4, 30), (4, 3, 3, 17),
-                                        (5, 2, 3, 11), (6, 3, 4, 9), (7, 2, 5, 6), (8, 3, 3, 5),
-                                        (10, 2, 3, 5)])
-def test_matrix_core_stage_matches_vector_stage(cuda, storage, n, nx, nu, nv):
-    """The BF16/FP16 stage kernel on the matrix cores (operator split into
-    hi + lo 16-bit parts, about 2^-17 relative, FP32 accumulation) against
-    the vector-ALU kernel (FP32 operator) after 3 resident steps.  The
-    operators differ below the storage precision, so an intermediate stage
-    value now and then rounds the other way, and later stages can cancel
-    such a value down to a small one.  Bars: fewer than 1% of the values
-    differ; no value differs by more than 4 storage ulps of the state's
-    maximum; 2^-13 relative L2 on the whole state (far below one storage
-    ulp).  The shapes cover every n, partial v-chunks and the
-    velocity-boundary pencils."""
-    import torch
+MX_CASES = [(9, 3, 4, 5), (2, 3, 3, 70), (3, 2, 4, 30), (4, 3, 3, 17), (5, 2, 3, 11), (6, 3, 4, 9),
+            (7, 2, 5, 6), (8, 3, 3, 5), (10, 2, 3, 5)]
+# row-split kernel: n = 8, 9 only; partial 4-element v-chunks (nv % 4 != 0),
+# one-element velocity grids (both u faces are ghosts), long x-pencils
+RS_CASES = [(9, 3, 4, 5), (9, 2, 1, 1), (9, 5, 3, 8), (8, 3, 3, 5), (8, 6, 2, 7), (9, 9, 2, 3)]
+
+
+def _three_resident_steps(storage, n, nx, nu, nv, mfma, layout):
     from vlasov1d2v_amd import Run, _lib
     from vlasov1d2v_amd.grid import STORAGE_CODES
-    runs = []
-    for mfma in (0, 1):
-        r = Run(n, nx, nu, nv, dt=0.004, storage=storage)
-        r.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
-        r.resident_begin()
-        for _ in range(3):
-            r.step()
-        r.resident_end()
-        runs.append(r)
+    r = Run(n, nx, nu, nv, dt=0.004, storage=storage)
+    r.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
+    r.resident_begin(layout=layout)
+    for _ in range(3):
+        r.step()
+    r.resident_end()
+    return r
+
+
+def _compare_16bit_runs(storage, n, ref_run, run, label):
+    """Bars for two 16-bit runs whose operators differ below the storage
+    precision: an intermediate stage value now and then rounds the other
+    way, and later stages can cancel such a value down to a small one.
+    Fewer than 1% of the values differ; no value differs by more than 4
+    storage ulps of the state's maximum; 2^-13 relative L2 on the whole
+    state (far below one storage ulp); the fields agree to 1e-4."""
+    import torch
     torch.cuda.synchronize()
-    d = ulp_distance(runs[1].dist.arr, runs[0].dist.arr)
-    g0 = runs[0].dist.arr.float().cpu().numpy()
-    g1 = runs[1].dist.arr.float().cpu().numpy()
+    d = ulp_distance(run.dist.arr, ref_run.dist.arr)
+    g0 = ref_run.dist.arr.float().cpu().numpy()
+    g1 = run.dist.arr.float().cpu().numpy()
     err = rel_l2(g1, g0)
     ulp = 2.0 ** -8 if storage == "bf16" else 2.0 ** -11
     worst = float(np.max(np.abs(g1 - g0))) / float(np.max(np.abs(g0)))
-    print(f"{storage} n={n}: differing fraction {np.mean(d > 0):.2e}, max |diff| {worst / ulp:.2f} ulp of max, "
-          f"rel-L2 {err:.2e}", flush=True)
+    print(f"{label} {storage} n={n}: differing fraction {np.mean(d > 0):.2e}, max |diff| {worst / ulp:.2f} "
+          f"ulp of max, rel-L2 {err:.2e}", flush=True)
     assert np.all(np.isfinite(g1))
     assert np.mean(d > 0) < 1e-2
     assert worst <= 4 * ulp
     assert err < 2.0 ** -13
-    e0 = runs[0].elliptic.electric_field.cpu().numpy()
-    e1 = runs[1].elliptic.electric_field.cpu().numpy()
+    e0 = ref_run.elliptic.electric_field.cpu().numpy()
+    e1 = run.elliptic.electric_field.cpu().numpy()
     assert rel_l2(e1, e0) < 1e-4
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("n,nx,nu,nv", MX_CASES)
+def test_matrix_core_stage_matches_vector_stage(cuda, storage, n, nx, nu, nv):
+    """The element-major BF16/FP16 stage kernel on the matrix cores (operator
+    split into hi + lo 16-bit parts, about 2^-17 relative, FP32
+    accumulation) against the vector-ALU kernel (FP32 operator) after 3
+    resident steps.  The shapes cover every n, partial v-chunks and the
+    velocity-boundary pencils."""
+    from vlasov1d2v_amd import _lib
+    em = _lib.VL_LAYOUT_ELEMENT_MAJOR
+    ref = _three_resident_steps(storage, n, nx, nu, nv, 0, em)
+    run = _three_resident_steps(storage, n, nx, nu, nv, 1, em)
+    _compare_16bit_runs(storage, n, ref, run, "element-major MFMA")
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("n,nx,nu,nv", RS_CASES)
+def test_row_split_stage_matches_vector_stage(cuda, storage, n, nx, nu, nv):
+    """The row-split BF16/FP16 stage kernel (transposes on the matrix cores,
+    upwind faces by byte permutation, background images as accumulator
+    inputs) against the element-major vector-ALU kernel, 3 resident steps."""
+    from vlasov1d2v_amd import _lib
+    ref = _three_resident_steps(storage, n, nx, nu, nv, 0, _lib.VL_LAYOUT_ELEMENT_MAJOR)
+    run = _three_resident_steps(storage, n, nx, nu, nv, 1, _lib.VL_LAYOUT_ROW_SPLIT)
+    _compare_16bit_runs(storage, n, ref, run, "row-split")
+
+
+@pytest.mark.parametrize("n", [8, 9])
+def test_row_split_layout_conversion(cuda, n):
+    """Reference -> row-split -> reference is exact, and the row-split array
+    holds node (ix, iu, iv) of element (ex, eu, ev) at
+    element * pitch + (iv < 8 ? (ix n + iu) 8 + iv : 8 n^2 + (ix n + iu) (n - 8) + iv - 8)."""
+    import torch
+    from vlasov1d2v_amd import Run, _lib
+    nx, nu, nv = 2, 3, 2
+    r = Run(n, nx, nu, nv, storage="bf16")
+    g = r.grids.geometry(_lib.VL_BF16)
+    shape = r.grids.shape
+    vals = (torch.arange(int(np.prod(shape)), dtype=torch.int32) % 30000).to(torch.int16)
+    src = vals.view(torch.bfloat16).reshape(shape).to(cuda)
+    rs = torch.zeros(g.state_elems_layout(_lib.VL_LAYOUT_ROW_SPLIT), dtype=torch.bfloat16, device=cuda)
+    g.convert_layout(src, _lib.VL_LAYOUT_REFERENCE, rs, _lib.VL_LAYOUT_ROW_SPLIT)
+    back = torch.zeros_like(src)
+    g.convert_layout(rs, _lib.VL_LAYOUT_ROW_SPLIT, back, _lib.VL_LAYOUT_REFERENCE)
+    torch.cuda.synchronize()
+    assert torch.equal(back.view(torch.int16).cpu(), src.view(torch.int16).cpu())
+    pitch = (n ** 3 + 7) // 8 * 8
+    a = src.view(torch.int16).cpu().numpy()
+    b = rs.view(torch.int16).cpu().numpy()
+    ex, ix, eu, iu, ev, iv = np.indices(shape).reshape(6, -1)
+    elem = (ex * (nu + 2) + eu) * (nv + 2) + ev
+    inner = np.where(iv < 8, (ix * n + iu) * 8 + iv, 8 * n * n + (ix * n + iu) * (n - 8) + iv - 8)
+    assert np.array_equal(b[elem * pitch + inner], a.reshape(-1))
+
+
+def test_resident_layout_choice(cuda):
+    """The library reports the row-split layout for the 16-bit forms with
+    n = 8, 9 and the element-major layout otherwise."""
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    for n, storage, want in [(9, "bf16", _lib.VL_LAYOUT_ROW_SPLIT), (8, "f16", _lib.VL_LAYOUT_ROW_SPLIT),
+                             (7, "bf16", _lib.VL_LAYOUT_ELEMENT_MAJOR), (9, "f32", _lib.VL_LAYOUT_ELEMENT_MAJOR),
+                             (9, "f64", _lib.VL_LAYOUT_ELEMENT_MAJOR)]:
+        r = Run(n, 2, 2, 2, storage=storage)
+        g = r.grids.geometry(STORAGE_CODES[storage])
+        assert g.resident_layout() == want, (n, storage)
+        if want == _lib.VL_LAYOUT_ROW_SPLIT:
+            g.set_tuning(_lib.VL_TUNE_ROW_SPLIT, 0)
+            assert g.resident_layout() == _lib.VL_LAYOUT_ELEMENT_MAJOR

@@ -48,6 +48,27 @@ def test_reduced_storage_initial_state_is_exact_background_plus_deviation(cuda):
 
 
 @pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_row_split_path_is_as_accurate_as_vector_path(cuda, storage):
+    """Same check for the row-split matrix-core kernel (n = 9, the default
+    resident layout of the 16-bit forms) against the element-major
+    vector-ALU kernel, both measured against the FP64 path."""
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    f64, e64 = _run("f64", n=9, nx=6, nu=8, nv=9, steps=12, dt=0.003)
+    errs = {}
+    for mfma in (0, 1):
+        run = Run(9, 6, 8, 9, seed=3, dt=0.003, storage=storage)
+        run.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
+        st = run.run(12)
+        assert run.stepper._res["layout"] == (_lib.VL_LAYOUT_ROW_SPLIT if mfma else _lib.VL_LAYOUT_ELEMENT_MAJOR)
+        f = run.dist.total().cpu().numpy()[1:-1, :, 1:-1, :, 1:-1, :]
+        errs[mfma] = (rel_l2(f, f64), rel_l2(st.field_energy, e64))
+    print(f"{storage}: vector {errs[0]}, row-split {errs[1]}", flush=True)
+    assert errs[1][0] <= 1.1 * errs[0][0]
+    assert errs[1][1] <= 1.1 * errs[0][1] + 1e-7
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
 def test_matrix_core_path_is_as_accurate_as_vector_path(cuda, storage):
     """Against the FP64 path, the matrix-core stage kernel (the default for
     the 16-bit forms in the resident layout) is as accurate as the

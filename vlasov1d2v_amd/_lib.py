@@ -20,12 +20,14 @@ EXPORTS = (
     "vl_rhs", "vl_moment0", "vl_poisson", "vl_field_energy", "vl_max_abs_acc_e",
     "vl_swap_ghosts", "vl_rk3_step", "vl_set_background", "vl_set_physics", "vl_set_stage_events",
     "vl_workspace_init", "vl_set_tuning", "vl_get_tuning", "vl_state_elems_layout",
-    "vl_convert_layout", "vl_last_error_string",
+    "vl_convert_layout", "vl_resident_layout", "vl_last_error_string",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
+VL_TUNE_ROW_SPLIT = 3
 VL_STEP_ELEMENT_MAJOR = 2
-VL_LAYOUT_REFERENCE, VL_LAYOUT_ELEMENT_MAJOR = 0, 1
+VL_STEP_ROW_SPLIT = 4
+VL_LAYOUT_REFERENCE, VL_LAYOUT_ELEMENT_MAJOR, VL_LAYOUT_ROW_SPLIT = 0, 1, 2
 
 
 class VlGeomDesc(ctypes.Structure):
@@ -79,6 +81,7 @@ def load(path=LIB_PATH):
         "vl_get_tuning": [vp, i32, ctypes.POINTER(i32)],
         "vl_state_elems_layout": [vp, i32, ctypes.POINTER(sz)],
         "vl_convert_layout": [vp, vp, i32, vp, i32, vp],
+        "vl_resident_layout": [vp, ctypes.POINTER(i32)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -183,6 +186,12 @@ class Geometry:
     def convert_layout(self, src, src_layout, dst, dst_layout):
         check(load().vl_convert_layout(self.handle, ptr(src), int(src_layout), ptr(dst), int(dst_layout),
                                        stream_ptr()), "vl_convert_layout")
+
+    def resident_layout(self):
+        """The layout vl_rk3_step runs fastest in (VL_LAYOUT_*)."""
+        v = ctypes.c_int()
+        check(load().vl_resident_layout(self.handle, ctypes.byref(v)), "vl_resident_layout")
+        return int(v.value)
 
     def get_tuning(self, key):
         v = ctypes.c_int()

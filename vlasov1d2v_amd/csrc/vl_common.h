@@ -25,6 +25,9 @@ struct OpTables {
     //   eoM[i] = up[i][m],  eoR[j] = up[m][j]          (i, j < h)
     double eoP[(kMaxN / 2) * (kMaxN / 2)], eoQ[(kMaxN / 2) * (kMaxN / 2)], eoM[kMaxN / 2], eoR[kMaxN / 2];
     float eoPf[(kMaxN / 2) * (kMaxN / 2)], eoQf[(kMaxN / 2) * (kMaxN / 2)], eoMf[kMaxN / 2], eoRf[kMaxN / 2];
+    // factored moment weights of the 16-bit forms: wq[iu][iv] ~ wu[iu] * wv[iv]
+    float wu[kMaxN];               // w_iu / (Ju Jv)
+    float wv[kMaxN];               // w_iv
 };
 
 // Raw buffer resource over `bytes` bytes from p (gfx9 data-format word).
@@ -190,6 +193,9 @@ struct StageArgs {
     double2 *partials;             // [(ex-1)*tiles_per_ex + tile][ix] (hi, lo)
     // reduced-precision forms: stored g = (f - f0) * scale
     const double *bg;              // [3][(nu+2)*n][(nv+2)*n]: G+, G-, Hv background tables
+    // same index space, FP32, operator images without speed or Jacobian:
+    // scale * (A+ f0 along u, A- f0 along u, A f0 along v upwinded by a_v)
+    const float *bg32;
     double scale, inv_scale;       // power of two
 };
 

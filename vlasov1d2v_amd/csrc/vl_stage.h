@@ -705,11 +705,13 @@ __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store
 }
 
 // Layout conversion of a state array (element width W bytes via type U):
-// to_elem = 1: reference -> element-major, 0: element-major -> reference.
-// Padding values of the element-major array are not written.
+// to_elem = 1: reference -> `lay`, 0: `lay` -> reference, where `lay` is
+// kElemLayout (element-major) or 2 (row-split, n = 8 or 9).  Padding values
+// of the element-wise arrays are not written.
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__ src, U *__restrict__ dst,
-                                                           int to_elem, int n, int nu, int nv, int64_t total) {
+                                                           int to_elem, int lay, int n, int nu, int nv,
+                                                           int64_t total) {
     const int64_t ep = elem_pitch(n);
     const int64_t sxe_e = (int64_t)(nu + 2) * (nv + 2) * ep;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
@@ -724,7 +726,10 @@ __global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__
         t /= (nu + 2);
         const int ix = (int)(t % n);
         const int64_t ex = t / n;
-        const int64_t e = ex * sxe_e + ((int64_t)eu * (nv + 2) + ev) * ep + (ix * n + iu) * n + iv;
+        const int inner = lay == kElemLayout ? (ix * n + iu) * n + iv
+                                             : (iv < 8 ? (ix * n + iu) * 8 + iv
+                                                       : 8 * n * n + (ix * n + iu) * (n - 8) + (iv - 8));
+        const int64_t e = ex * sxe_e + ((int64_t)eu * (nv + 2) + ev) * ep + inner;
         if (to_elem)
             dst[e] = src[i];
         else

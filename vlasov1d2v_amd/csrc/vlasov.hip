@@ -16,6 +16,7 @@
 #include "vl_field.h"
 #include "vl_stage.h"
 #include "vl_stage_mx.h"
+#include "vl_stage_rs.h"
 
 #include <vector>
 
@@ -28,6 +29,8 @@ struct vl_geom {
     int TV, nchunk, tiles_per_ex, tiles;
     int xseg, pencils;     // stage kernel: x elements per pencil segment, pencils launched
     int mfma;              // 1: matrix-core stage kernel for the 16-bit forms (element-major)
+    int row_split;         // 1: the 16-bit forms with n = 8, 9 prefer the row-split layout
+    int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: 4 v-elements per tile
     int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
@@ -35,6 +38,7 @@ struct vl_geom {
     double *d_unodes, *d_vnodes, *d_wts, *d_kwave;
     double2 *d_T, *d_phases;
     double *d_bg;          // reduced forms: G+, G-, Hv background tables
+    float *d_bg32;         // reduced forms: scaled operator images of f0 (row-split kernel)
     OpTables *d_tab;       // device copy of tab (read by the stage kernel)
     std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
     double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
@@ -92,7 +96,7 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     s.nx = h->nx;
     s.nu = h->nu;
     s.nv = h->nv;
-    s.nchunk = h->nchunk;
+    s.nchunk = layout == kRowSplitLayout ? h->nchunk4 : h->nchunk;
     s.xseg = h->xseg;
     s.jx = h->jx;
     s.ju = h->ju;
@@ -100,10 +104,11 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     s.acc = h->acc;
     s.bfield = h->bfield;
     s.sx = h->sx;
-    s.sxe = layout == kElemLayout ? h->sxe_elem : h->sxe;
+    s.sxe = layout != kRefLayout ? h->sxe_elem : h->sxe;
     s.su = h->su;
     s.sue = h->sue;
     s.bg = h->d_bg;
+    s.bg32 = h->d_bg32;
     s.scale = h->scale;
     s.inv_scale = h->inv_scale;
     return s;
@@ -163,6 +168,23 @@ hipError_t launch_stage_mx(const vl_geom *h, const void *fin, const void *fgh, c
     VL_DISPATCH_N((launch_stage_mx_ns<NN, MODE, S>(h, fin, fgh, fn, fout, E, s, st)))
 }
 
+template <int N, int MODE, int S>
+hipError_t launch_stage_rs_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                              const double *E, const StageArgs &s, hipStream_t st) {
+    hipLaunchKernelGGL((k_stage_rs<N, MODE, S>), dim3(h->pencils4), dim3(kBlock), 0, st, (const uint16_t *)fin,
+                       (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
+                       h->d_vnodes, s, h->d_tab);
+    return hipGetLastError();
+}
+
+template <int MODE, int S>
+hipError_t launch_stage_rs(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
+                           const double *E, const StageArgs &s, hipStream_t st) {
+    if (h->n == 8) return launch_stage_rs_ns<8, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+    if (h->n == 9) return launch_stage_rs_ns<9, MODE, S>(h, fin, fgh, fn, fout, E, s, st);
+    return hipErrorInvalidValue;
+}
+
 template <int MODE, int S, int L>
 hipError_t launch_stage_sl(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                            const double *E, const StageArgs &s, hipStream_t st) {
@@ -177,6 +199,11 @@ hipError_t launch_stage(const vl_geom *h, int storage, int layout, const void *f
 #define VL_STAGE_S(SS)                                                                                  \
     return layout == kElemLayout ? launch_stage_sl<MODE, SS, kElemLayout>(h, fin, fgh, fn, fout, E, s, st) \
                                  : launch_stage_sl<MODE, SS, kRefLayout>(h, fin, fgh, fn, fout, E, s, st)
+    if (layout == kRowSplitLayout) {
+        if (storage == VL_BF16) return launch_stage_rs<MODE, VL_BF16>(h, fin, fgh, fn, fout, E, s, st);
+        if (storage == VL_F16) return launch_stage_rs<MODE, VL_F16>(h, fin, fgh, fn, fout, E, s, st);
+        return hipErrorInvalidValue;
+    }
     if (layout == kElemLayout && h->mfma) {
         if (storage == VL_BF16) return launch_stage_mx<MODE, VL_BF16>(h, fin, fgh, fn, fout, E, s, st);
         if (storage == VL_F16) return launch_stage_mx<MODE, VL_F16>(h, fin, fgh, fn, fout, E, s, st);
@@ -211,8 +238,22 @@ hipError_t launch_moment_sl(const vl_geom *h, const void *f, const StageArgs &s,
     VL_DISPATCH_N((launch_moment_nsl<NN, S, L>(h, f, s, st)))
 }
 
+template <int N, int S>
+hipError_t launch_moment_rs_ns(const vl_geom *h, const void *f, const StageArgs &s, hipStream_t st) {
+    hipLaunchKernelGGL((k_moment_rs<N, S>), dim3(h->tiles4), dim3(kBlock), 0, st, (const uint16_t *)f, s,
+                       h->d_tab);
+    return hipGetLastError();
+}
+
 hipError_t launch_moment(const vl_geom *h, int storage, int layout, const void *f, const StageArgs &s,
                          hipStream_t st) {
+    if (layout == kRowSplitLayout) {
+        if (storage == VL_BF16 && h->n == 9) return launch_moment_rs_ns<9, VL_BF16>(h, f, s, st);
+        if (storage == VL_BF16 && h->n == 8) return launch_moment_rs_ns<8, VL_BF16>(h, f, s, st);
+        if (storage == VL_F16 && h->n == 9) return launch_moment_rs_ns<9, VL_F16>(h, f, s, st);
+        if (storage == VL_F16 && h->n == 8) return launch_moment_rs_ns<8, VL_F16>(h, f, s, st);
+        return hipErrorInvalidValue;
+    }
 #define VL_MOM_S(SS)                                                                   \
     return layout == kElemLayout ? launch_moment_sl<SS, kElemLayout>(h, f, s, st) \
                                  : launch_moment_sl<SS, kRefLayout>(h, f, s, st)
@@ -246,23 +287,28 @@ unsigned int *ws_ticket(const vl_geom *h, void *ws) {
 }
 
 template <int N>
-hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, hipStream_t st) {
-    hipLaunchKernelGGL((k_reduce_partials<N>), dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws),
-                       h->tiles_per_ex, rho);
+hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, int tpe, hipStream_t st) {
+    hipLaunchKernelGGL((k_reduce_partials<N>), dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws), tpe, rho);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st) {
+// partial tiles per x element of a layout's stage / moment kernels
+int tiles_per_ex_of(const vl_geom *h, int layout) {
+    return layout == kRowSplitLayout ? h->tiles_per_ex4 : h->tiles_per_ex;
+}
+
+hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st, int layout = kRefLayout) {
+    const int tpe = tiles_per_ex_of(h, layout);
     switch (h->n) {
-        case 2: return launch_reduce_n<2>(h, ws, rho, st);
-        case 3: return launch_reduce_n<3>(h, ws, rho, st);
-        case 4: return launch_reduce_n<4>(h, ws, rho, st);
-        case 5: return launch_reduce_n<5>(h, ws, rho, st);
-        case 6: return launch_reduce_n<6>(h, ws, rho, st);
-        case 7: return launch_reduce_n<7>(h, ws, rho, st);
-        case 8: return launch_reduce_n<8>(h, ws, rho, st);
-        case 9: return launch_reduce_n<9>(h, ws, rho, st);
-        case 10: return launch_reduce_n<10>(h, ws, rho, st);
+        case 2: return launch_reduce_n<2>(h, ws, rho, tpe, st);
+        case 3: return launch_reduce_n<3>(h, ws, rho, tpe, st);
+        case 4: return launch_reduce_n<4>(h, ws, rho, tpe, st);
+        case 5: return launch_reduce_n<5>(h, ws, rho, tpe, st);
+        case 6: return launch_reduce_n<6>(h, ws, rho, tpe, st);
+        case 7: return launch_reduce_n<7>(h, ws, rho, tpe, st);
+        case 8: return launch_reduce_n<8>(h, ws, rho, tpe, st);
+        case 9: return launch_reduce_n<9>(h, ws, rho, tpe, st);
+        case 10: return launch_reduce_n<10>(h, ws, rho, tpe, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -285,8 +331,8 @@ hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho,
 
 // partials -> rho (workspace) -> E (+phi, energy)
 hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *phi, double *energy,
-                               hipStream_t st) {
-    hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st);
+                               hipStream_t st, int layout) {
+    hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st, layout);
     if (e != hipSuccess) return e;
     return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st);
 }
@@ -319,6 +365,9 @@ int upload_background(vl_geom *h) {
     const int64_t W = (int64_t)(nv + 2) * n;           // row length (v nodes)
     const int64_t tbl = (int64_t)(nu + 2) * n * W;
     std::vector<double> bg((size_t)(3 * tbl), 0.0);
+    // FP32 tables of the row-split kernel: scale * (A+ f0, A- f0) along u and
+    // scale * A f0 along v (upwinded by the sign of a_v), no speed or Jacobian
+    std::vector<float> bg32((size_t)(3 * tbl), 0.0f);
     const double *f0 = h->f0_host.data();
     auto F0 = [&](int eu, int iu, int64_t gv) { return f0[((int64_t)eu * n + iu) * W + gv]; };
     std::vector<double> unodes((size_t)(nu + 2) * n);
@@ -335,6 +384,8 @@ int upload_background(vl_geom *h) {
                     const int64_t o = ((int64_t)eu * n + i) * W + gv;
                     bg[o] = h->ju * gp;
                     bg[tbl + o] = h->ju * gm;
+                    bg32[o] = (float)(h->scale * gp);
+                    bg32[tbl + o] = (float)(h->scale * gm);
                 }
             }
     for (int eu = 1; eu <= nu; ++eu)
@@ -352,11 +403,14 @@ int upload_background(vl_geom *h) {
                     const double face = pos ? (h->tab.xi0[i] * fl_nb - h->tab.xi1[i] * fr[n - 1])
                                             : (h->tab.xi0[i] * fr[0] - h->tab.xi1[i] * fr_nb);
                     bg[2 * tbl + row + (int64_t)ev * n + i] = h->jv * av * (vol + face);
+                    bg32[2 * tbl + row + (int64_t)ev * n + i] = (float)(h->scale * (vol + face));
                 }
             }
         }
     if (!h->d_bg) VL_HIP(hipMalloc((void **)&h->d_bg, bg.size() * sizeof(double)));
     VL_HIP(hipMemcpy(h->d_bg, bg.data(), bg.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!h->d_bg32) VL_HIP(hipMalloc((void **)&h->d_bg32, bg32.size() * sizeof(float)));
+    VL_HIP(hipMemcpy(h->d_bg32, bg32.data(), bg32.size() * sizeof(float), hipMemcpyHostToDevice));
     return VL_OK;
 }
 
@@ -441,6 +495,10 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     for (int iu = 0; iu < n; ++iu)
         for (int iv = 0; iv < n; ++iv)
             h->tab.wq[iu * n + iv] = (d->weights[iu] * d->weights[iv]) / (d->ju * d->jv);
+    for (int i = 0; i < n; ++i) {
+        h->tab.wu[i] = (float)(d->weights[i] / (d->ju * d->jv));
+        h->tab.wv[i] = (float)d->weights[i];
+    }
     h->TV = tile_tv(n);
     h->nchunk = (d->nv + h->TV - 1) / h->TV;
     h->tiles_per_ex = d->nu * h->nchunk;
@@ -461,6 +519,12 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         h->mfma = 1;
         h->pencils = base * ((d->nx + h->xseg - 1) / h->xseg);
     }
+    // row-split kernels: tiles of 4 v-elements (same x-segments)
+    h->row_split = 1;
+    h->nchunk4 = (d->nv + 3) / 4;
+    h->tiles_per_ex4 = d->nu * h->nchunk4;
+    h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
+    h->pencils4 = h->tiles_per_ex4 * ((d->nx + h->xseg - 1) / h->xseg);
     h->su = (int64_t)(d->nv + 2) * n;
     h->sue = h->su * n;
     h->sx = h->sue * (d->nu + 2);
@@ -475,7 +539,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
     size_t off = 0;
     h->off_partials = off;
-    off = align_up(off + (size_t)tiles * n * sizeof(double2), 256);
+    off = align_up(off + (size_t)std::max<int64_t>(tiles, h->tiles4) * n * sizeof(double2), 256);
     h->off_rho = off;
     off = align_up(off + (size_t)d->nx * n * sizeof(double), 256);
     h->off_coef = off;
@@ -510,7 +574,8 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
 int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
-    void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts, h->d_kwave, h->d_T, h->d_phases, h->d_bg, h->d_tab};
+    void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts,   h->d_kwave, h->d_T,
+                    h->d_phases, h->d_bg,     h->d_bg32, h->d_tab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -596,7 +661,7 @@ int vl_max_abs_acc_e(vl_geom_t h, const double *E, double *out, void *stream) {
 
 int swap_ghosts_layout(vl_geom_t h, void *f, int layout, void *stream) {
     const size_t eb = storage_bytes(h->storage);
-    const int64_t plane = layout == kElemLayout ? h->sxe_elem : h->sxe;
+    const int64_t plane = layout != kRefLayout ? h->sxe_elem : h->sxe;
     const int64_t blocks = std::min<int64_t>((2 * plane + kBlock - 1) / kBlock, 8192);
     hipStream_t st = (hipStream_t)stream;
     if (eb == 8)
@@ -629,9 +694,14 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
         return fail(VL_E_ARG, "vl_rk3_step: reduced storage needs vl_set_background first");
     void *f = fv, *f1 = f1v, *f2 = f2v;
     const int S = h->storage;
-    const int LAY = (flags & VL_STEP_ELEMENT_MAJOR) ? kElemLayout : kRefLayout;
-    if (LAY == kElemLayout && (!aligned(fv, 16) || !aligned(f1v, 16) || !aligned(f2v, 16)))
-        return fail(VL_E_ALIGN, "vl_rk3_step: element-major arrays must be 16-byte aligned");
+    if ((flags & VL_STEP_ELEMENT_MAJOR) && (flags & VL_STEP_ROW_SPLIT))
+        return fail(VL_E_ARG, "vl_rk3_step: VL_STEP_ELEMENT_MAJOR and VL_STEP_ROW_SPLIT are exclusive");
+    const int LAY = (flags & VL_STEP_ROW_SPLIT) ? kRowSplitLayout
+                    : (flags & VL_STEP_ELEMENT_MAJOR) ? kElemLayout : kRefLayout;
+    if (LAY != kRefLayout && (!aligned(fv, 16) || !aligned(f1v, 16) || !aligned(f2v, 16)))
+        return fail(VL_E_ALIGN, "vl_rk3_step: element-wise arrays must be 16-byte aligned");
+    if (LAY == kRowSplitLayout && ((h->storage != VL_BF16 && h->storage != VL_F16) || (h->n != 8 && h->n != 9)))
+        return fail(VL_E_ARG, "vl_rk3_step: the row-split layout needs BF16/FP16 storage and n = 8 or 9");
     hipStream_t st = (hipStream_t)stream;
     if (int r = swap_ghosts_layout(h, f, LAY, stream)) return r;
     StageArgs s = make_stage_args(h, LAY);
@@ -640,7 +710,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.partials = ws_partials(h, ws);
     if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, LAY, f, s, st));
     // stage 1: f1 = f + dt L(f)                       (main/timestep.py:153-163)
-    VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
+    VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st, LAY));
     s.c0 = 0.0;
     s.c1 = 1.0;
     s.c2dt = dt;
@@ -648,7 +718,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     VL_HIP(launch_stage<kModeStage1>(h, S, LAY, f, f, nullptr, f1, E, s, st));
     VL_HIP(record_stage_event(h, 1, st));
     // stage 2: f2 = 3/4 f + 1/4 f1 + 1/4 dt L(f1)     (main/timestep.py:165-182)
-    VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st));
+    VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st, LAY));
     s.c0 = 3.0 / 4.0;
     s.c1 = 1.0 / 4.0;
     s.c2dt = (1.0 / 4.0) * dt;
@@ -656,7 +726,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     VL_HIP(launch_stage<kModeStage>(h, S, LAY, f1, f, f, f2, E, s, st));
     VL_HIP(record_stage_event(h, 3, st));
     // stage 3: f = 1/3 f + 2/3 f2 + 2/3 dt L(f2); E, phi, energy of f2's field
-    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st));
+    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st, LAY));
     s.c0 = 1.0 / 3.0;
     s.c1 = 2.0 / 3.0;
     s.c2dt = (2.0 / 3.0) * dt;
@@ -669,8 +739,11 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
 int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems) {
     if (int r = check_handle(h)) return r;
     if (!elems) return fail(VL_E_NULL, "null elems");
-    if (layout != VL_LAYOUT_REFERENCE && layout != VL_LAYOUT_ELEMENT_MAJOR) return fail(VL_E_ARG, "bad layout");
-    *elems = layout == VL_LAYOUT_ELEMENT_MAJOR ? (size_t)h->sxe_elem * (h->nx + 2) : h->state_elems;
+    if (layout != VL_LAYOUT_REFERENCE && layout != VL_LAYOUT_ELEMENT_MAJOR && layout != VL_LAYOUT_ROW_SPLIT)
+        return fail(VL_E_ARG, "bad layout %d", layout);
+    if (layout == VL_LAYOUT_ROW_SPLIT && h->n != 8 && h->n != 9)
+        return fail(VL_E_ARG, "the row-split layout needs n = 8 or 9 (n = %d)", h->n);
+    *elems = layout != VL_LAYOUT_REFERENCE ? (size_t)h->sxe_elem * (h->nx + 2) : h->state_elems;
     return VL_OK;
 }
 
@@ -678,9 +751,13 @@ int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, i
     if (int r = check_handle(h)) return r;
     if (!src || !dst) return fail(VL_E_NULL, "vl_convert_layout: null pointer");
     if (src == dst) return fail(VL_E_ARG, "vl_convert_layout: in-place conversion not supported");
-    const bool to_elem = src_layout == VL_LAYOUT_REFERENCE && dst_layout == VL_LAYOUT_ELEMENT_MAJOR;
-    const bool from_elem = src_layout == VL_LAYOUT_ELEMENT_MAJOR && dst_layout == VL_LAYOUT_REFERENCE;
+    auto elementwise = [](int l) { return l == VL_LAYOUT_ELEMENT_MAJOR || l == VL_LAYOUT_ROW_SPLIT; };
+    const bool to_elem = src_layout == VL_LAYOUT_REFERENCE && elementwise(dst_layout);
+    const bool from_elem = elementwise(src_layout) && dst_layout == VL_LAYOUT_REFERENCE;
     if (!to_elem && !from_elem) return fail(VL_E_ARG, "vl_convert_layout: unsupported layout pair");
+    const int lay = to_elem ? dst_layout : src_layout;
+    if (lay == VL_LAYOUT_ROW_SPLIT && h->n != 8 && h->n != 9)
+        return fail(VL_E_ARG, "the row-split layout needs n = 8 or 9 (n = %d)", h->n);
     const size_t eb = storage_bytes(h->storage);
     if (!aligned(src, eb) || !aligned(dst, eb)) return fail(VL_E_ALIGN, "vl_convert_layout: misaligned");
     const int64_t total = (int64_t)h->state_elems;
@@ -689,13 +766,13 @@ int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, i
     const int te = to_elem ? 1 : 0;
     if (eb == 8)
         hipLaunchKernelGGL(k_convert_layout<double>, dim3(blocks), dim3(kBlock), 0, st, (const double *)src,
-                           (double *)dst, te, h->n, h->nu, h->nv, total);
+                           (double *)dst, te, lay, h->n, h->nu, h->nv, total);
     else if (eb == 4)
         hipLaunchKernelGGL(k_convert_layout<float>, dim3(blocks), dim3(kBlock), 0, st, (const float *)src,
-                           (float *)dst, te, h->n, h->nu, h->nv, total);
+                           (float *)dst, te, lay, h->n, h->nu, h->nv, total);
     else
         hipLaunchKernelGGL(k_convert_layout<uint16_t>, dim3(blocks), dim3(kBlock), 0, st,
-                           (const uint16_t *)src, (uint16_t *)dst, te, h->n, h->nu, h->nv, total);
+                           (const uint16_t *)src, (uint16_t *)dst, te, lay, h->n, h->nu, h->nv, total);
     VL_HIP(hipGetLastError());
     return VL_OK;
 }
@@ -715,6 +792,12 @@ int vl_set_tuning(vl_geom_t h, int key, int value) {
         if (value < 1) return fail(VL_E_ARG, "xseg must be >= 1");
         h->xseg = std::min(value, h->nx);
         h->pencils = h->nu * h->nchunk * ((h->nx + h->xseg - 1) / h->xseg);
+        h->pencils4 = h->tiles_per_ex4 * ((h->nx + h->xseg - 1) / h->xseg);
+        return VL_OK;
+    }
+    if (key == VL_TUNE_ROW_SPLIT) {
+        if (value != 0 && value != 1) return fail(VL_E_ARG, "row_split must be 0 or 1");
+        h->row_split = value;
         return VL_OK;
     }
     if (key == VL_TUNE_MFMA) {
@@ -736,7 +819,20 @@ int vl_get_tuning(vl_geom_t h, int key, int *value) {
         *value = h->mfma;
         return VL_OK;
     }
+    if (key == VL_TUNE_ROW_SPLIT) {
+        *value = h->row_split;
+        return VL_OK;
+    }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
+}
+
+int vl_resident_layout(vl_geom_t h, int *layout) {
+    if (int r = check_handle(h)) return r;
+    if (!layout) return fail(VL_E_NULL, "vl_resident_layout: null layout");
+    const bool rs = (h->storage == VL_BF16 || h->storage == VL_F16) && (h->n == 8 || h->n == 9) && h->mfma &&
+                    h->row_split;
+    *layout = rs ? VL_LAYOUT_ROW_SPLIT : VL_LAYOUT_ELEMENT_MAJOR;
+    return VL_OK;
 }
 
 int vl_set_stage_events(vl_geom_t h, void **events) {

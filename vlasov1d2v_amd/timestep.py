@@ -89,31 +89,34 @@ class Stepper:
                              grids.x.dx, grids.u.dx, grids.v.dx)
 
     # -- resident element-major stepping ------------------------------------
-    def resident_begin(self, distribution, grids, dg_flux, elliptic):
-        """Copy distribution.arr into the element-major layout (each element's
-        n^3 nodes contiguous) and step there until resident_end(); the
+    def resident_begin(self, distribution, grids, dg_flux, elliptic, layout=None):
+        """Copy distribution.arr into an element-wise layout (each element's
+        n^3 nodes contiguous; by default the one the library reports fastest
+        for this storage and n) and step there until resident_end(); the
         reference-layout distribution.arr is stale in between."""
         import torch
         arr = distribution.arr
         _check_state(arr, grids.shape, "resident_begin", allow_reduced=True)
         g = dg_flux._geom(grids, elliptic, STORAGE_CODES[storage_of(arr)])
-        n_em = g.state_elems_layout(_lib.VL_LAYOUT_ELEMENT_MAJOR)
+        if layout is None:
+            layout = g.resident_layout()
+        n_em = g.state_elems_layout(layout)
         r = self._res
         if r is None or r["bufs"][0].numel() != n_em or r["bufs"][0].dtype != arr.dtype \
                 or r["bufs"][0].device != arr.device:
             bufs = tuple(torch.zeros(n_em, dtype=arr.dtype, device=arr.device) for _ in range(3))
         else:
             bufs = r["bufs"]
-        g.convert_layout(arr, _lib.VL_LAYOUT_REFERENCE, bufs[0], _lib.VL_LAYOUT_ELEMENT_MAJOR)
-        self._res = {"bufs": bufs, "dist": distribution, "geom": g, "token": None, "active": True}
+        g.convert_layout(arr, _lib.VL_LAYOUT_REFERENCE, bufs[0], layout)
+        self._res = {"bufs": bufs, "dist": distribution, "geom": g, "token": None, "active": True,
+                     "layout": layout}
 
     def resident_end(self, distribution):
         """Write the resident state back into distribution.arr (reference layout)."""
         r = self._res
         if r is None or not r["active"]:
             return
-        r["geom"].convert_layout(r["bufs"][0], _lib.VL_LAYOUT_ELEMENT_MAJOR, distribution.arr,
-                                 _lib.VL_LAYOUT_REFERENCE)
+        r["geom"].convert_layout(r["bufs"][0], r["layout"], distribution.arr, _lib.VL_LAYOUT_REFERENCE)
         r["active"] = False
         self._rho_token = None
 
@@ -145,7 +148,8 @@ class Stepper:
         token = (arr.data_ptr(), arr._version, ws.data_ptr())
         flags = _lib.VL_STEP_RHO_VALID if token == self._rho_token else 0
         if res is not None:
-            flags |= _lib.VL_STEP_ELEMENT_MAJOR
+            flags |= (_lib.VL_STEP_ROW_SPLIT if res["layout"] == _lib.VL_LAYOUT_ROW_SPLIT
+                      else _lib.VL_STEP_ELEMENT_MAJOR)
         if self.dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
         _lib.check(_lib.load().vl_rk3_step(

@@ -89,8 +89,8 @@ class Run:
         self.stepper.nonlinear_ssp_rk(self.dist, self.basis, self.elliptic, self.grids, self.flux,
                                       energy_out=energy_out)
 
-    def resident_begin(self):
-        self.stepper.resident_begin(self.dist, self.grids, self.flux, self.elliptic)
+    def resident_begin(self, layout=None):
+        self.stepper.resident_begin(self.dist, self.grids, self.flux, self.elliptic, layout=layout)
 
     def resident_end(self):
         self.stepper.resident_end(self.dist)
