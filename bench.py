@@ -58,6 +58,8 @@ def main():
                     help="storage form(s) to time; the headline value is the FP64 form")
     ap.add_argument("--layout", default="elem", choices=["elem", "ref"],
                     help="resident element-major layout (default) or the reference layout")
+    ap.add_argument("--xseg", type=int, default=0,
+                    help="x elements each stage-kernel workgroup walks (0: the library default)")
     ap.add_argument("--no-mfma", action="store_true",
                     help="run the BF16/FP16 stages on the vector ALUs instead of the matrix cores")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -88,7 +90,8 @@ def main():
     forms = {}
     storages = ["f64", "f32", "bf16", "f16"] if args.storage == "all" else [args.storage]
     for storage in storages:
-        forms[storage] = measure(w, storage, args.steps, args.warmup, args.layout, dev, mfma=not args.no_mfma)
+        forms[storage] = measure(w, storage, args.steps, args.warmup, args.layout, dev, mfma=not args.no_mfma,
+                                 xseg=args.xseg)
         torch.cuda.empty_cache()
     head = forms["f64"] if "f64" in forms else forms[storages[0]]
     N = w.dof
@@ -153,7 +156,7 @@ def profiled_traffic(kernel, storage):
     return (fetch * cal[0] + write * cal[1]) * 1024.0
 
 
-def measure(w, storage, K, W, layout, dev, mfma=True):
+def measure(w, storage, K, W, layout, dev, mfma=True, xseg=0):
     """Time K SSP-RK3 steps (after W warmup steps) of one storage form."""
     import ctypes
     import torch
@@ -163,6 +166,8 @@ def measure(w, storage, K, W, layout, dev, mfma=True):
     run = Run.from_workload(w, device=dev, storage=storage)
     g = run.grids.geometry(STORAGE_CODES[storage])
     g.set_tuning(_lib.VL_TUNE_MFMA, 1 if mfma else 0)
+    if xseg > 0:
+        g.set_tuning(_lib.VL_TUNE_XSEG, xseg)
     energies = torch.empty(W + K + 1, dtype=torch.float64, device=dev)
     if layout == "elem":
         run.resident_begin()

@@ -103,7 +103,8 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
 
 /* Launch-shape tuning of the stage kernel (no counterpart in the reference).
  * VL_TUNE_XSEG: x elements each workgroup walks (default: chosen so that the
- * launch has about 13000 workgroups).
+ * launch has about 13000 workgroups, about 5200 for the row-split kernel;
+ * a set value applies to both).
  * VL_TUNE_MFMA: 1 (default) runs the BF16/FP16 stages of the element-major
  * layout on the matrix cores, 0 on the vector ALUs.
  * VL_TUNE_ROW_SPLIT: 1 (default) makes vl_resident_layout report the

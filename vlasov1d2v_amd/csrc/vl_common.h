@@ -196,6 +196,9 @@ struct StageArgs {
     // same index space, FP32, operator images without speed or Jacobian:
     // scale * (A+ f0 along u, A- f0 along u, A f0 along v upwinded by a_v)
     const float *bg32;
+    // matrix-core operator fragments of the row-split kernel: [format][lane][4]
+    // (format 0 BF16, 1 FP16; words hi.x, hi.y, lo.x, lo.y)
+    const unsigned *mxfrag;
     double scale, inv_scale;       // power of two
 };
 

@@ -19,9 +19,12 @@
 namespace vl {
 
 constexpr int kFieldBlock = 1024;
-constexpr int kDftModes = 16;     // modes per workgroup of the forward DFT
-constexpr int kInvNodes = 64;     // nodes per workgroup of the inverse sum
-constexpr int kInvGroups = 4;     // mode groups per node in the inverse sum
+// The spectral tables (2 x nx*n*M complex) are read once per stage; the
+// launches below spread them over ~80 workgroups with several independent
+// loads in flight per thread, so that the solve costs a few load latencies.
+constexpr int kDftModes = 4;      // modes per workgroup of the forward DFT
+constexpr int kInvNodes = 8;      // nodes per workgroup of the inverse sum
+constexpr int kInvGroups = 32;    // mode groups per node in the inverse sum
 
 // Block-wide double-double sum over blockDim.x (a power of two) threads;
 // returns the rounded value to every thread.
@@ -151,25 +154,23 @@ __global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict
     const int m = blockIdx.x * kDftModes + mm;
     double re = 0.0, im = 0.0;
     if (m < a.M) {
-        int k = g;
-        for (; k + 3 * G < K; k += 4 * G) {
-            const double2 t0 = T[(int64_t)k * a.M + m];
-            const double2 t1 = T[(int64_t)(k + G) * a.M + m];
-            const double2 t2 = T[(int64_t)(k + 2 * G) * a.M + m];
-            const double2 t3 = T[(int64_t)(k + 3 * G) * a.M + m];
-            re += rhs[k] * t0.x;
-            im += rhs[k] * t0.y;
-            re += rhs[k + G] * t1.x;
-            im += rhs[k + G] * t1.y;
-            re += rhs[k + 2 * G] * t2.x;
-            im += rhs[k + 2 * G] * t2.y;
-            re += rhs[k + 3 * G] * t3.x;
-            im += rhs[k + 3 * G] * t3.y;
-        }
-        for (; k < K; k += G) {
-            const double2 t0 = T[(int64_t)k * a.M + m];
-            re += rhs[k] * t0.x;
-            im += rhs[k] * t0.y;
+        // batches of 8 independent table loads, then the multiply-adds in k order
+        constexpr int U = 8;
+        for (int k0 = g; k0 < K; k0 += U * G) {
+            double2 t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * G;
+                t[u] = k < K ? T[(int64_t)k * a.M + m] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * G;
+                if (k < K) {
+                    re += rhs[k] * t[u].x;
+                    im += rhs[k] * t[u].y;
+                }
+            }
         }
     }
     // fixed-order tree over the G groups of each mode
@@ -222,13 +223,26 @@ __global__ __launch_bounds__(kBlock) void k_dft_inverse(
     const int k = blockIdx.x * kInvNodes + kk;
     double e = 0.0, p = 0.0;
     if (k < K) {
-        for (int m = g; m < M; m += kInvGroups) {
-            const double2 ph = phases[(int64_t)m * K + k];
-            const double2 c = cdyn[m];
-            e += c.x * ph.x - c.y * ph.y;
-            if (phi) {
-                const double2 cp = cdyn[M + m];
-                p += cp.x * ph.x - cp.y * ph.y;
+        // batches of 8 independent phase loads, then the sums in m order
+        constexpr int U = 8;
+        for (int m0 = g; m0 < M; m0 += U * kInvGroups) {
+            double2 ph[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int m = m0 + u * kInvGroups;
+                ph[u] = m < M ? phases[(int64_t)m * K + k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int m = m0 + u * kInvGroups;
+                if (m < M) {
+                    const double2 c = cdyn[m];
+                    e += c.x * ph[u].x - c.y * ph[u].y;
+                    if (phi) {
+                        const double2 cp = cdyn[M + m];
+                        p += cp.x * ph[u].x - cp.y * ph[u].y;
+                    }
+                }
             }
         }
     }

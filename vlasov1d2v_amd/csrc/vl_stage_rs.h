@@ -44,8 +44,9 @@
 // scale * A_v f0, are the accumulator inputs of those MFMAs.
 //
 // Work decomposition.  A workgroup of 4 waves owns a pencil (eu, 4
-// consecutive v-elements, x-segment) and walks it along x.  Waves are
-// independent except for one barrier per tile for the moment partials.
+// consecutive v-elements, x-segment) and walks it along x.  The waves are
+// independent (no barriers): each writes the moment partial of its own
+// element, one slot per tile element.
 // The global loads of the next tile are in flight while the current one is
 // computed; the epilogue works on whole rows (16-byte loads and stores).
 #pragma once
@@ -54,6 +55,21 @@
 namespace vl {
 
 enum { kRowSplitLayout = 2 };
+
+// Waves per SIMD the row-split stage kernel is compiled for (register budget
+// 512 / VL_RS_WAVES per lane).
+#ifndef VL_RS_WAVES
+#define VL_RS_WAVES 4
+#endif
+// 1: no instruction scheduling across the x / u-v tasks of one tile
+#ifndef VL_RS_TASK_FENCE
+#define VL_RS_TASK_FENCE 0
+#endif
+__device__ __forceinline__ void task_fence() {
+#if VL_RS_TASK_FENCE
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
 
 template <int N>
 struct RsShape {
@@ -109,13 +125,15 @@ template <int N>
 struct RsLoads {
     vl_u32x4 row[RsShape<N>::NRI];   // own rows: nodes 0..7 of line (ix, iu)
     unsigned tail[RsShape<N>::NRI];  // own node 8 (n = 9)
-    unsigned vnb[RsShape<N>::NRI];   // v-neighbour faces of the row: nbL | nbR << 16
+    // v-neighbour faces of the row (kept apart until the brick is filled, so
+    // that no arithmetic waits for loads of a tile that is still prefetching)
+    unsigned vnl[RsShape<N>::NRI], vnr[RsShape<N>::NRI];
     vl_u32x4 nrow;                   // one neighbour row (x or u face plane)
     unsigned ntail;
 };
 
 template <int N, int MODE, int S>
-__global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
+__global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const uint16_t *__restrict__ fin,   // stage input
     const uint16_t *__restrict__ fgh,   // array whose u/v ghost elements are used
     const uint16_t *fn,                 // f^n for MODE == kModeStage (may alias fout)
@@ -133,11 +151,13 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
 
     __shared__ __attribute__((aligned(16))) unsigned int brick[TV * ESD];
     __shared__ __attribute__((aligned(16))) float xa[TV * XE];
-    __shared__ float red[2][TV * ROWS];
+    __shared__ float red[TV * ROWS];   // per-row moment sums, wave-private rows
+    // per-lane sink of the 16-byte stores of lanes that hold no output node
+    __shared__ __attribute__((aligned(16))) float sink[TV * 64 * 4];
 
     const int tid = threadIdx.x;
     const int ln = tid & 63;
-    const int el = tid >> 6;            // this wave's element of the tile
+    const int el = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's element (wave-uniform)
     const int lj = ln & 15;             // MFMA column / row within a 16 x 16 block
     const int lg = ln >> 4;             // lane group: K slots / output rows 4 lg .. 4 lg + 3
     const Pencil pc = decode_pencil(s);
@@ -159,34 +179,25 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
     };
 
     // ---- per-wave constants ------------------------------------------------
-    // operator fragments: lane holds op[i = lj][slot 4 lg + t], hi and lo parts
+    // operator fragments (host-built): lane holds op[i = lj][slot 4 lg + t],
+    // hi and lo 16-bit parts
     vl_u32x2 ophi, oplo, idv;
     {
-        unsigned h[4], l[4], id[4];
+        const vl_u32x4 fr = *reinterpret_cast<const vl_u32x4 *>(s.mxfrag + ((S == 2 ? 0 : 64) + ln) * 4);
+        ophi = vl_u32x2{fr.x, fr.y};
+        oplo = vl_u32x2{fr.z, fr.w};
         const unsigned one = F::enc(1.0f);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int sl = 4 * lg + t;
-            double c = 0.0;
-            if (lj < N) {
-                if (sl < 8) c = tabp->up[lj * N + sl];
-                else if (sl == 8) c = N == 9 ? tabp->up[lj * N + 8] : 0.0;
-                else if (sl == 10) c = tabp->xi0[lj];
-                else if (sl == 11) c = -tabp->xi1[lj];
-            }
-            h[t] = F::enc((float)c);
-            l[t] = F::enc((float)(c - (double)F::dec(h[t])));
-            id[t] = sl == lj ? one : 0u;
-        }
-        ophi = vl_u32x2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
-        oplo = vl_u32x2{l[0] | (l[1] << 16), l[2] | (l[3] << 16)};
-        idv = vl_u32x2{id[0] | (id[1] << 16), id[2] | (id[3] << 16)};
+        const int d = lj - 4 * lg;   // identity: 1 at slot lj
+        idv = vl_u32x2{(d == 0 ? one : 0u) | ((d == 1 ? one : 0u) << 16),
+                       (d == 2 ? one : 0u) | ((d == 3 ? one : 0u) << 16)};
     }
     // background images (accumulator inputs), v-speed terms: lane column
     // iu = lj, rows iv = 4 lg + t of this wave's element
-    vl_f32x4 gp = {0.f, 0.f, 0.f, 0.f}, gm = gp, hv = gp, pt = gp;
-    float pl = 0.f, sv = 0.f;
+    // u background as average + half difference: su * G(sign su) = su Gavg + |su| Gh
+    vl_f32x4 gavg = {0.f, 0.f, 0.f, 0.f}, gh = gavg, hv = gavg, pt = gavg;
+    float pl = 0.f, sv = 0.f, sxl = 0.f;
     unsigned vsel = kSelKeep;
+    unsigned xposm = 0u;               // bit su: u(eu, su) >= 0 (x-line upwinding), wave-uniform
     const float jua = (float)(s.ju * s.acc);
     if (active) {
         const int64_t W = (int64_t)vrow * N;
@@ -198,8 +209,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
                 pt[t] = (float)(s.ju * s.acc * (s.bfield * vnodes[ev * N + iv]));
                 if (lj < N) {
                     const int64_t o = ((int64_t)eu * N + lj) * W + (int64_t)ev * N + iv;
-                    gp[t] = s.bg32[o];
-                    gm[t] = s.bg32[tbl + o];
+                    const float gp = s.bg32[o], gm = s.bg32[tbl + o];
+                    gavg[t] = 0.5f * (gp + gm);
+                    gh[t] = 0.5f * (gp - gm);
                     hv[t] = s.bg32[2 * tbl + o];
                 }
             }
@@ -208,40 +220,68 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
         pl = (float)(s.ju * s.acc * (s.bfield * vnodes[ev * N + jc]));
         const double av = s.acc * (-s.bfield * unodes[eu * N + jc]);   // main/fluxes.py:143
         sv = (float)(s.jv * av);
+        sxl = (float)(s.jx * unodes[eu * N + jc]);                      // main/fluxes.py:110
         if (lg == 2) vsel = av >= 0.0 ? kSelPos : kSelNeg;
+#pragma unroll
+        for (int su = 0; su < N; ++su) xposm |= (unodes[eu * N + su] >= 0.0 ? 1u : 0u) << su;
         // never-written brick slots stay zero (finite) for the whole kernel
         for (int i = ln; i < ESD; i += 64) bk[i] = 0u;
     }
 
     // ---- global loads of one tile ------------------------------------------
+    // Raw buffer loads over whole state arrays (fewer than 2^31 bytes, checked
+    // by the host): uniform resource + 32-bit byte offset per lane; offsets
+    // of lanes with nothing to load are kOob and read 0 (no branches).
+    const int sbytes = (int)((int64_t)(s.nx + 2) * sxe * 2);
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin, sbytes);
+    const __amdgpu_buffer_rsrc_t rgh = make_rsrc(fgh, sbytes);
+    const __amdgpu_buffer_rsrc_t rfn = make_rsrc(fn, sbytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc(fout, sbytes);
+    auto ld16 = [&](__amdgpu_buffer_rsrc_t rr, int off) {
+        return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(rr, off, 0, 0);
+    };
+    // Single 16-bit values of the prefetch are read as the aligned dword that
+    // holds them (offset & ~3) and picked at fill time; a 16-bit load would be
+    // zero-extended at once, stalling on a load meant to stay in flight.
+    auto ld32 = [&](__amdgpu_buffer_rsrc_t rr, int off) {
+        return (unsigned)__builtin_amdgcn_raw_buffer_load_b32(rr, off, 0, 0);
+    };
+    // neighbour rows: lane -> (side, j); sides 0/1 x-left/right, 2/3 u-left/right
+    const int nside = ln / N, nj = ln - (ln / N) * N;
+    const bool nlane = ln < NNB;
+    const int nrr = nside == 0 ? (N - 1) * N + nj : nside == 1 ? nj : nside == 2 ? nj * N + (N - 1) : nj * N;
+    const __amdgpu_buffer_rsrc_t ruL = eu - 1 == 0 ? rgh : rin;        // wave-uniform
+    const __amdgpu_buffer_rsrc_t ruR = eu + 1 == s.nu + 1 ? rgh : rin;
     auto load = [&](int ex, RsLoads<N> &d) {
         if (!active) return;
-        const int64_t b = ebase(ex, eu, ev);
-        const uint16_t *pL = ev - 1 == 0 ? fgh : fin;
-        const uint16_t *pR = ev + 1 == s.nv + 1 ? fgh : fin;
-        const int64_t bl = ebase(ex, eu, ev - 1), br = ebase(ex, eu, ev + 1);
+        const int b = (int)(ebase(ex, eu, ev) * 2);
+        const __amdgpu_buffer_rsrc_t rL = ev - 1 == 0 ? rgh : rin;
+        const __amdgpu_buffer_rsrc_t rR = ev + 1 == s.nv + 1 ? rgh : rin;
+        const int bl = (int)(ebase(ex, eu, ev - 1) * 2), br = (int)(ebase(ex, eu, ev + 1) * 2);
 #pragma unroll
         for (int k = 0; k < NRI; ++k) {
             const int r = ln + 64 * k;
-            if (r < ROWS) {
-                d.row[k] = *reinterpret_cast<const vl_u32x4 *>(fin + b + r * 8);
-                d.tail[k] = N == 9 ? (unsigned)fin[b + 8 * ROWS + r] : 0u;
-                const unsigned nl = N == 9 ? (unsigned)pL[bl + 8 * ROWS + r] : (unsigned)pL[bl + r * 8 + 7];
-                const unsigned nr = (unsigned)pR[br + r * 8];
-                d.vnb[k] = nl | (nr << 16);
-            }
+            const bool ok = r < ROWS;
+            d.row[k] = buf_ld16(rin, ok ? b + r * 16 : kOob);
+            d.tail[k] = N == 9 ? ld32(rin, ok ? (b + (8 * ROWS + r) * 2) & ~3 : kOob) : 0u;
+            d.vnl[k] = ld32(rL, ok ? (bl + (N == 9 ? (8 * ROWS + r) * 2 : r * 16 + 12)) & ~3 : kOob);
+            d.vnr[k] = ld32(rR, ok ? br + r * 16 : kOob);
         }
-        if (ln < NNB) {
-            const int side = ln / N, j = ln - (ln / N) * N;
-            int exn = ex, eun = eu, rr;
-            const uint16_t *p = fin;
-            if (side == 0) { exn = xmap(ex - 1); rr = (N - 1) * N + j; }        // x-left:  ix = n-1, iu = j
-            else if (side == 1) { exn = xmap(ex + 1); rr = j; }                 // x-right: ix = 0,   iu = j
-            else if (side == 2) { eun = eu - 1; rr = j * N + (N - 1); if (eun == 0) p = fgh; }        // u-left
-            else { eun = eu + 1; rr = j * N; if (eun == s.nu + 1) p = fgh; }                          // u-right
-            const int64_t bn = ebase(exn, eun, ev);
-            d.nrow = *reinterpret_cast<const vl_u32x4 *>(p + bn + rr * 8);
-            d.ntail = N == 9 ? (unsigned)p[bn + 8 * ROWS + rr] : 0u;
+        const int exn = nside == 0 ? xmap(ex - 1) : nside == 1 ? xmap(ex + 1) : ex;
+        const int eun = nside == 2 ? eu - 1 : nside == 3 ? eu + 1 : eu;
+        const int bn = (int)(ebase(exn, eun, ev) * 2);
+        const int orow = nlane ? bn + nrr * 16 : kOob, otail = nlane ? (bn + (8 * ROWS + nrr) * 2) & ~3 : kOob;
+        // the resource is uniform per side: u-neighbours of the first / last
+        // u-element are ghost elements, read from fgh
+        if (nside == 2) {
+            d.nrow = buf_ld16(ruL, orow);
+            d.ntail = N == 9 ? ld32(ruL, otail) : 0u;
+        } else if (nside == 3) {
+            d.nrow = buf_ld16(ruR, orow);
+            d.ntail = N == 9 ? ld32(ruR, otail) : 0u;
+        } else {
+            d.nrow = buf_ld16(rin, orow);
+            d.ntail = N == 9 ? ld32(rin, otail) : 0u;
         }
     };
     // registers -> brick (slots 0..11 of every row that the passes read)
@@ -254,28 +294,33 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
             if (faces) *reinterpret_cast<vl_u32x2 *>(p + 4) = vl_u32x2{w4, w5};
             else p[4] = w4;
         };
+        // halves ha of a and hb of b -> one word (a's in the low half)
+        auto halves = [](unsigned a, int ha, unsigned b, int hb) {
+            const unsigned sa = 2u * ha, sb = 4u + 2u * hb;
+            return __builtin_amdgcn_perm(b, a, sa | ((sa + 1) << 8) | (sb << 16) | ((sb + 1) << 24));
+        };
 #pragma unroll
         for (int k = 0; k < NRI; ++k) {
             const int r = ln + 64 * k;
             if (r < ROWS) {
                 const int ix = r / N, iu = r - (r / N) * N;
                 const vl_u32x4 R = d.row[k];
-                const unsigned last = N == 9 ? d.tail[k] : (R.w >> 16);
-                const unsigned w4 = (last & 0xffffu) | (R.x << 16);   // slot 8 = node n-1, slot 9 = node 0
-                put(ix, iu, R, w4, d.vnb[k], true);
-                if (iu == 0) put(ix, 9, R, w4, d.vnb[k], true);       // u-slot 9: copy of node 0
-                if (ix == 0) put(9, iu, R, w4, d.vnb[k], true);       // x-slot 9: copy of node 0
-                if (N == 8 && iu == N - 1) put(ix, 8, R, w4, d.vnb[k], true);   // slot 8: copy of node n-1
-                if (N == 8 && ix == N - 1) put(8, iu, R, w4, d.vnb[k], true);
+                // the 16-bit tail values sit in half (r & 1) of their dword (8 n^2 is even)
+                const unsigned w4 = N == 9 ? halves(d.tail[k], r & 1, R.x, 0)    // slot 8 = node n-1,
+                                           : halves(R.w, 1, R.x, 0);             // slot 9 = node 0
+                const unsigned w5 = halves(d.vnl[k], N == 9 ? (r & 1) : 1, d.vnr[k], 0);   // slots 10, 11: v faces
+                put(ix, iu, R, w4, w5, true);
+                if (iu == 0) put(ix, 9, R, w4, w5, true);       // u-slot 9: copy of node 0
+                if (ix == 0) put(9, iu, R, w4, w5, true);       // x-slot 9: copy of node 0
+                if (N == 8 && iu == N - 1) put(ix, 8, R, w4, w5, true);   // slot 8: copy of node n-1
+                if (N == 8 && ix == N - 1) put(8, iu, R, w4, w5, true);
             }
         }
-        if (ln < NNB) {
-            const int side = ln / N, j = ln - (ln / N) * N;
-            const int xs = side == 0 ? 10 : side == 1 ? 11 : j;
-            const int us = side == 2 ? 10 : side == 3 ? 11 : j;
+        if (nlane) {
+            const int xs = nside == 0 ? 10 : nside == 1 ? 11 : nj;
+            const int us = nside == 2 ? 10 : nside == 3 ? 11 : nj;
             const vl_u32x4 R = d.nrow;
-            const unsigned last = N == 9 ? d.ntail : (R.w >> 16);
-            put(xs, us, R, (last & 0xffffu) | (R.x << 16), 0u, false);
+            put(xs, us, R, N == 9 ? halves(d.ntail, nrr & 1, R.x, 0) : halves(R.w, 1, R.x, 0), 0u, false);
         }
     };
 
@@ -284,160 +329,163 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_rs(
     const bool partials = s.write_partials != 0;
     const vl_f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
 
+    // Per tile: fill the brick from `cur`, prefetch the next tile into `nxt`,
+    // compute, store.
     RsLoads<N> cur, nxt;
     load(pc.ex_begin, cur);
-    int par = 0;
     for (int ex = pc.ex_begin; ex <= pc.ex_end; ++ex) {
         const bool has_next = ex < pc.ex_end;
         fill(cur);
-        wave_sync_lds();
         if (has_next) load(ex + 1, nxt);
         const int64_t b = ebase(ex, eu, ev);
-        vl_u32x4 fnr[NRI];
-        unsigned fnt[NRI];
-        if (MODE == kModeStage && active) {
-#pragma unroll
-            for (int k = 0; k < NRI; ++k) {
-                const int r = ln + 64 * k;
-                if (r < ROWS) {
-                    fnr[k] = *reinterpret_cast<const vl_u32x4 *>(fn + b + r * 8);
-                    fnt[k] = N == 9 ? (unsigned)fn[b + 8 * ROWS + r] : 0u;
-                }
-            }
-        }
+        wave_sync_lds();
         if (active) {
-            // ---- x: X[ix][iu][v-slot] = Jx u(iu) (A_x g) ----------------------
+            // Branch-free passes: lanes beyond the 12 slots read a clamped
+            // (finite) brick row -- it only meets zero operator columns or
+            // feeds output lines nobody keeps -- and lanes without an output
+            // node store to their sink.
+            const int rdx = min(lj, 11) * PSD + 2 * min(lg, 2);    // x-slot lj, v-slots 4 lg ..
+            const int rduv = min(lj, 11) * RSD + 2 * min(lg, 2);   // u-slot lj, v-slots 4 lg ..
+            const bool wr = lj < N && lg <= GMAX;
+            float *const snk = sink + (el * 64 + ln) * 4;
+            const unsigned selp = lg == 2 ? kSelPos : kSelKeep, seln = lg == 2 ? kSelNeg : kSelKeep;
+            // ---- x: X[ix][iu][v-slot] = A_x g (the x speed is applied in the u/v pass)
 #pragma unroll
             for (int su = 0; su < N; ++su) {
-                const double uu = unodes[eu * N + su];
-                const float sxs = (float)(s.jx * uu);                       // main/fluxes.py:110
-                const unsigned sel = lg == 2 ? (uu >= 0.0 ? kSelPos : kSelNeg) : kSelKeep;
-                vl_u32x2 r = {0u, 0u};
-                if (lj < 12 && lg < 3) r = *reinterpret_cast<const vl_u32x2 *>(bk + lj * PSD + su * RSD + 2 * lg);
+                const vl_u32x2 r = *reinterpret_cast<const vl_u32x2 *>(bk + rdx + su * RSD);
                 const vl_f32x4 tr = F::mfma(r, idv, zero4);   // lane: v-slot lj, x-slots 4 lg + t
                 vl_u32x2 p = {PK::exact2(tr[0], tr[1]), PK::exact2(tr[2], tr[3])};
-                p.y = __builtin_amdgcn_perm(p.x, p.y, sel);
+                p.y = __builtin_amdgcn_perm(p.x, p.y, ((xposm >> su) & 1u) ? selp : seln);
                 vl_f32x4 d = F::mfma(p, ophi, zero4);
                 d = F::mfma(p, oplo, d);                      // lane: x-node lj, v-slots 4 lg + t
-                if (lj < N && lg <= GMAX)
-                    *reinterpret_cast<vl_f32x4 *>(xw + (lj * N + su) * XR + 4 * lg) = d * sxs;
+                float *dst = wr ? xw + (lj * N + su) * XR + 4 * lg : snk;
+                *reinterpret_cast<vl_f32x4 *>(dst) = d;
+                task_fence();
             }
+            // E of this tile's x nodes, as the u-speed offset Ju acc E (uniform)
+            float qs[N];
+#pragma unroll
+            for (int sx = 0; sx < N; ++sx)
+                qs[sx] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, jua * (float)E[ex * N + sx])));
             wave_sync_lds();
-            // ---- u and v: X += Ju a_u (A_u g + G+-) + Jv a_v (A_v g + Hv) -----
+            // ---- u and v: L = Jx a_x X + Ju a_u (A_u g + G+-) + Jv a_v (A_v g + Hv)
 #pragma unroll
             for (int sx = 0; sx < N; ++sx) {
-                const float q = jua * (float)E[ex * N + sx];
-                vl_u32x2 r = {0u, 0u};
-                if (lj < 12 && lg < 3) r = *reinterpret_cast<const vl_u32x2 *>(bk + sx * PSD + lj * RSD + 2 * lg);
+                const float q = qs[sx];
+                const vl_u32x2 r = *reinterpret_cast<const vl_u32x2 *>(bk + sx * PSD + rduv);
                 vl_u32x2 bvv = r;
                 bvv.y = __builtin_amdgcn_perm(r.x, r.y, vsel);
                 vl_f32x4 dv = F::mfma(ophi, bvv, hv);
                 dv = F::mfma(oplo, bvv, dv);                  // lane: u-slot lj, v-nodes 4 lg + t
                 const vl_f32x4 tr = F::mfma(r, idv, zero4);   // lane: v-slot lj, u-slots 4 lg + t
                 vl_u32x2 p = {PK::exact2(tr[0], tr[1]), PK::exact2(tr[2], tr[3])};
-                const float sl = q + pl;                       // Ju a_u of this lane's u-line
-                p.y = __builtin_amdgcn_perm(p.x, p.y, lg == 2 ? (sl >= 0.f ? kSelPos : kSelNeg) : kSelKeep);
-                vl_f32x4 su4, gsel;
+                p.y = __builtin_amdgcn_perm(p.x, p.y, (q + pl >= 0.f) ? selp : seln);   // Ju a_u of the lane's line
+                vl_f32x4 du = F::mfma(p, ophi, gavg);
+                du = F::mfma(p, oplo, du);                    // lane: u-node lj, v-slots 4 lg + t
+                float *xp = wr ? xw + (sx * N + lj) * XR + 4 * lg : snk;
+                const vl_f32x4 xv = *reinterpret_cast<const vl_f32x4 *>(xp);
+                vl_f32x4 L;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    su4[t] = q + pt[t];
-                    gsel[t] = su4[t] >= 0.f ? gp[t] : gm[t];
+                    const float su = q + pt[t];
+                    L[t] = __builtin_fmaf(sv, dv[t],
+                                          __builtin_fmaf(su, du[t], __builtin_fmaf(__builtin_fabsf(su), gh[t], sxl * xv[t])));
                 }
-                vl_f32x4 du = F::mfma(p, ophi, gsel);
-                du = F::mfma(p, oplo, du);                    // lane: u-node lj, v-slots 4 lg + t
-                if (lj < N && lg <= GMAX) {
-                    float *xp = xw + (sx * N + lj) * XR + 4 * lg;
-                    const vl_f32x4 xv = *reinterpret_cast<const vl_f32x4 *>(xp);
-                    vl_f32x4 L;
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) L[t] = __builtin_fmaf(sv, dv[t], __builtin_fmaf(su4[t], du[t], xv[t]));
-                    *reinterpret_cast<vl_f32x4 *>(xp) = L;
-                }
+                *reinterpret_cast<vl_f32x4 *>(xp) = L;
+                task_fence();
             }
             wave_sync_lds();
             // ---- epilogue: whole rows ----------------------------------------
             const int64_t bo = ebase(ex, eu, ev);
-#pragma unroll
+#pragma unroll 1
             for (int k = 0; k < NRI; ++k) {
                 const int r = ln + 64 * k;
                 if (r < ROWS) {
-                    const int iu = r - (r / N) * N;
-                    const float *xp = xw + r * XR;
-                    const vl_f32x4 l0 = *reinterpret_cast<const vl_f32x4 *>(xp);
-                    const vl_f32x4 l1 = *reinterpret_cast<const vl_f32x4 *>(xp + 4);
-                    float Lr[9] = {l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3], N == 9 ? xp[8] : 0.f};
-                    const vl_u32x4 R = cur.row[k];
-                    const unsigned rw[4] = {R.x, R.y, R.z, R.w};
-                    float fi[9], fv[9];
-#pragma unroll
-                    for (int h = 0; h < 4; ++h) {
-                        fi[2 * h] = PK::lo(rw[h]);
-                        fi[2 * h + 1] = PK::hi(rw[h]);
-                    }
-                    fi[8] = N == 9 ? PK::lo(cur.tail[k]) : 0.f;
+                    vl_u32x4 fnr = {0u, 0u, 0u, 0u};
+                    unsigned fnt = 0u;
                     if (MODE == kModeStage) {
-                        const unsigned nw[4] = {fnr[k].x, fnr[k].y, fnr[k].z, fnr[k].w};
-#pragma unroll
-                        for (int h = 0; h < 4; ++h) {
-                            fv[2 * h] = PK::lo(nw[h]);
-                            fv[2 * h + 1] = PK::hi(nw[h]);
-                        }
-                        fv[8] = N == 9 ? PK::lo(fnt[k]) : 0.f;
+                        fnr = buf_ld16(rfn, (int)(b * 2) + r * 16);
+                        fnt = N == 9 ? ld32(rfn, ((int)(b * 2) + (8 * ROWS + r) * 2) & ~3) : 0u;
                     }
-                    float o[9];
-#pragma unroll
-                    for (int j = 0; j < 9; ++j)
-                        o[j] = MODE == kModeStage1 ? fi[j] + c2 * Lr[j] : (c0 * fv[j] + c1 * fi[j]) + c2 * Lr[j];
+                    const int iu = r - (r / N) * N;
+                    const int ix = r / N;
+                    const float *xp = xw + r * XR;
+                    const unsigned *br = bk + ix * PSD + iu * RSD;    // stage input row, slots 0..8
+                    const unsigned nw[4] = {fnr.x, fnr.y, fnr.z, fnr.w};
+                    ConstOpTables &tab = *(ConstOpTables *)tabp;
+                    // pairs of values: combine, round, moment of the stored pair
+                    auto pair = [&](float l0, float l1, unsigned fiw, unsigned fnw, float &m, int j) {
+                        float o0, o1;
+                        if (MODE == kModeStage1) {
+                            o0 = PK::lo(fiw) + c2 * l0;
+                            o1 = PK::hi(fiw) + c2 * l1;
+                        } else {
+                            o0 = (c0 * PK::lo(fnw) + c1 * PK::lo(fiw)) + c2 * l0;
+                            o1 = (c0 * PK::hi(fnw) + c1 * PK::hi(fiw)) + c2 * l1;
+                        }
+                        const unsigned wv2 = F::enc2(o0, o1);
+                        m += tab.wv[j] * PK::lo(wv2);
+                        m += tab.wv[j + 1] * PK::hi(wv2);
+                        return wv2;
+                    };
+                    float m = 0.f;
                     vl_u32x4 w;
-                    w.x = F::enc2(o[0], o[1]);
-                    w.y = F::enc2(o[2], o[3]);
-                    w.z = F::enc2(o[4], o[5]);
-                    w.w = F::enc2(o[6], o[7]);
-                    *reinterpret_cast<vl_u32x4 *>(fout + bo + r * 8) = w;
-                    unsigned t8 = 0u;
+                    {
+                        const vl_f32x4 l0 = *reinterpret_cast<const vl_f32x4 *>(xp);
+                        const vl_u32x2 f01 = *reinterpret_cast<const vl_u32x2 *>(br);
+                        w.x = pair(l0[0], l0[1], f01.x, nw[0], m, 0);
+                        w.y = pair(l0[2], l0[3], f01.y, nw[1], m, 2);
+                    }
+                    {
+                        const vl_f32x4 l1 = *reinterpret_cast<const vl_f32x4 *>(xp + 4);
+                        const vl_u32x2 f23 = *reinterpret_cast<const vl_u32x2 *>(br + 2);
+                        w.z = pair(l1[0], l1[1], f23.x, nw[2], m, 4);
+                        w.w = pair(l1[2], l1[3], f23.y, nw[3], m, 6);
+                    }
+                    buf_st16(rout, (int)(bo * 2) + r * 16, w);
                     if (N == 9) {
-                        t8 = F::enc(o[8]);
-                        fout[bo + 8 * ROWS + r] = (uint16_t)t8;
+                        const float fi8 = PK::lo(br[4]);
+                        const float fn8 = (r & 1) ? PK::hi(fnt) : PK::lo(fnt);
+                        const float o8 = MODE == kModeStage1 ? fi8 + c2 * xp[8] : (c0 * fn8 + c1 * fi8) + c2 * xp[8];
+                        const unsigned t8 = F::enc(o8);
+                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)t8, rout, (int)(bo * 2) + (8 * ROWS + r) * 2,
+                                                              0, 0);
+                        m += tab.wv[8] * PK::lo(t8);
                     }
                     // moment partial of the stored values (main/grid.py:319-321)
-                    const unsigned ww[4] = {w.x, w.y, w.z, w.w};
-                    ConstOpTables &tab = *(ConstOpTables *)tabp;
-                    float m = 0.f;
-#pragma unroll
-                    for (int h = 0; h < 4; ++h) {
-                        m += tab.wv[2 * h] * PK::lo(ww[h]);
-                        m += tab.wv[2 * h + 1] * PK::hi(ww[h]);
-                    }
-                    if (N == 9) m += tab.wv[8] * PK::lo(t8);
-                    red[par][el * ROWS + r] = tab.wu[iu] * m;
+                    red[el * ROWS + r] = tab.wu[iu] * m;
                 }
             }
         }
+        // moment partial of this wave's element (one slot per tile element,
+        // zero for the unused elements of a partial chunk); no barrier
         if (partials) {
-            __syncthreads();
-            if (tid < N) {
+            wave_sync_lds();
+            if (ln < N) {
                 double a = 0.0;
-                for (int e2 = 0; e2 < nvalid; ++e2)
-                    for (int iu = 0; iu < N; ++iu) a += (double)red[par][e2 * ROWS + tid * N + iu];
-                const int64_t slot = ((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * N + tid;
+                if (active)
+                    for (int iu = 0; iu < N; ++iu) a += (double)red[el * ROWS + ln * N + iu];
+                const int64_t slot =
+                    (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + ln;
                 s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
             }
         }
         cur = nxt;
-        par ^= 1;
     }
 }
 
 // Moment partials of a row-split state (first stage of a loop), in the
-// partial layout of k_stage_rs: one workgroup per tile (ex, eu, 4 v-elements).
+// partial layout of k_stage_rs (one slot per tile element): one workgroup per
+// tile (ex, eu, 4 v-elements), one wave per element.
 template <int N, int S>
 __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict__ f, StageArgs s,
                                                       const OpTables *__restrict__ tabp) {
     using SH = RsShape<N>;
     using PK = RsPack<S>;
     constexpr int TV = SH::TV, EP = SH::EP, ROWS = SH::ROWS, NRI = SH::NRI;
-    __shared__ float red[TV * ROWS];
-    const int tid = threadIdx.x, ln = tid & 63, el = tid >> 6;
+    __shared__ float red[TV * ROWS];   // wave-private rows
+    const int tid = threadIdx.x, ln = tid & 63, el = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile % s.nchunk;
     const int t2 = tile / s.nchunk;
@@ -465,13 +513,13 @@ __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict
             }
         }
     }
-    __syncthreads();
-    if (tid < N) {
+    wave_sync_lds();
+    if (ln < N) {
         double a = 0.0;
-        for (int e2 = 0; e2 < nvalid; ++e2)
-            for (int iu = 0; iu < N; ++iu) a += (double)red[e2 * ROWS + tid * N + iu];
+        if (el < nvalid)
+            for (int iu = 0; iu < N; ++iu) a += (double)red[el * ROWS + ln * N + iu];
         const int tiles_per_ex = s.nu * s.nchunk;
-        const int64_t slot = ((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * N + tid;
+        const int64_t slot = (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * TV + el) * N + ln;
         s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
     }
 }

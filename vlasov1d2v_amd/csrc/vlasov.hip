@@ -31,6 +31,7 @@ struct vl_geom {
     int mfma;              // 1: matrix-core stage kernel for the 16-bit forms (element-major)
     int row_split;         // 1: the 16-bit forms with n = 8, 9 prefer the row-split layout
     int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: 4 v-elements per tile
+    int xseg4;             // row-split stage kernel: x elements per pencil
     int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
@@ -39,6 +40,7 @@ struct vl_geom {
     double2 *d_T, *d_phases;
     double *d_bg;          // reduced forms: G+, G-, Hv background tables
     float *d_bg32;         // reduced forms: scaled operator images of f0 (row-split kernel)
+    unsigned *d_mxfrag;    // row-split kernel: operator fragments per lane (BF16, FP16)
     OpTables *d_tab;       // device copy of tab (read by the stage kernel)
     std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
     double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
@@ -97,7 +99,7 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     s.nu = h->nu;
     s.nv = h->nv;
     s.nchunk = layout == kRowSplitLayout ? h->nchunk4 : h->nchunk;
-    s.xseg = h->xseg;
+    s.xseg = layout == kRowSplitLayout ? h->xseg4 : h->xseg;
     s.jx = h->jx;
     s.ju = h->ju;
     s.jv = h->jv;
@@ -109,6 +111,7 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     s.sue = h->sue;
     s.bg = h->d_bg;
     s.bg32 = h->d_bg32;
+    s.mxfrag = h->d_mxfrag;
     s.scale = h->scale;
     s.inv_scale = h->inv_scale;
     return s;
@@ -294,7 +297,7 @@ hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, int tpe, hip
 
 // partial tiles per x element of a layout's stage / moment kernels
 int tiles_per_ex_of(const vl_geom *h, int layout) {
-    return layout == kRowSplitLayout ? h->tiles_per_ex4 : h->tiles_per_ex;
+    return layout == kRowSplitLayout ? h->tiles_per_ex4 * 4 : h->tiles_per_ex;   // row-split: one per element
 }
 
 hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st, int layout = kRefLayout) {
@@ -340,6 +343,63 @@ hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *ph
 hipError_t record_stage_event(const vl_geom *h, int idx, hipStream_t st) {
     if (!h->stage_events || !h->stage_events[idx]) return hipSuccess;
     return hipEventRecord((hipEvent_t)h->stage_events[idx], st);
+}
+
+// 16-bit encodings on the host, round to nearest even (finite inputs)
+uint16_t host_bf16(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+float host_bf16_dec(uint16_t h) {
+    const uint32_t b = (uint32_t)h << 16;
+    float x;
+    std::memcpy(&x, &b, 4);
+    return x;
+}
+uint16_t host_f16(float x) {
+    const _Float16 h = (_Float16)x;
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
+}
+float host_f16_dec(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return (float)h;
+}
+
+// Operator fragments of the row-split kernel: lane l (column lj = l % 16,
+// slot group lg = l / 16) holds op[lj][4 lg + t], t = 0..3, split into a
+// 16-bit hi part and a 16-bit lo part of the remainder.  Line slots: 0..7
+// nodes 0..7, 8 node n-1 (coefficient only when n = 9), 9 node 0 (copy, 0),
+// 10 left face (xi0), 11 right face (-xi1); rows >= n are zero.
+std::vector<unsigned> build_mxfrag(const OpTables &tab, int n) {
+    std::vector<unsigned> out(2 * 64 * 4, 0u);
+    for (int fmt = 0; fmt < 2; ++fmt)
+        for (int l = 0; l < 64; ++l) {
+            const int lj = l % 16, lg = l / 16;
+            uint16_t hi[4], lo[4];
+            for (int t = 0; t < 4; ++t) {
+                const int sl = 4 * lg + t;
+                double c = 0.0;
+                if (lj < n) {
+                    if (sl < 8 && sl < n) c = tab.up[lj * n + sl];
+                    else if (sl == 8) c = n == 9 ? tab.up[lj * n + 8] : 0.0;
+                    else if (sl == 10) c = tab.xi0[lj];
+                    else if (sl == 11) c = -tab.xi1[lj];
+                }
+                hi[t] = fmt == 0 ? host_bf16((float)c) : host_f16((float)c);
+                const double rem = c - (double)(fmt == 0 ? host_bf16_dec(hi[t]) : host_f16_dec(hi[t]));
+                lo[t] = fmt == 0 ? host_bf16((float)rem) : host_f16((float)rem);
+            }
+            unsigned *o = &out[(size_t)(fmt * 64 + l) * 4];
+            o[0] = hi[0] | ((unsigned)hi[1] << 16);
+            o[1] = hi[2] | ((unsigned)hi[3] << 16);
+            o[2] = lo[0] | ((unsigned)lo[1] << 16);
+            o[3] = lo[2] | ((unsigned)lo[3] << 16);
+        }
+    return out;
 }
 
 int check_handle(const vl_geom *h) {
@@ -524,7 +584,14 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     h->nchunk4 = (d->nv + 3) / 4;
     h->tiles_per_ex4 = d->nu * h->nchunk4;
     h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
-    h->pencils4 = h->tiles_per_ex4 * ((d->nx + h->xseg - 1) / h->xseg);
+    // about 5200 workgroups (8 x elements per pencil at 116.6M DOF: measured
+    // best among 4, 8, 16 on MI355X)
+    {
+        int nseg = (5200 + h->tiles_per_ex4 - 1) / h->tiles_per_ex4;
+        nseg = std::max(1, std::min(nseg, d->nx));
+        h->xseg4 = (d->nx + nseg - 1) / nseg;
+    }
+    h->pencils4 = h->tiles_per_ex4 * ((d->nx + h->xseg4 - 1) / h->xseg4);
     h->su = (int64_t)(d->nv + 2) * n;
     h->sue = h->su * n;
     h->sx = h->sue * (d->nu + 2);
@@ -539,7 +606,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
     size_t off = 0;
     h->off_partials = off;
-    off = align_up(off + (size_t)std::max<int64_t>(tiles, h->tiles4) * n * sizeof(double2), 256);
+    off = align_up(off + (size_t)std::max<int64_t>(tiles, 4 * (int64_t)h->tiles4) * n * sizeof(double2), 256);
     h->off_rho = off;
     off = align_up(off + (size_t)d->nx * n * sizeof(double), 256);
     h->off_coef = off;
@@ -556,6 +623,10 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     hipError_t e = hipGetDevice(&h->device);
     const int K = d->nx * n;
     if (e == hipSuccess) e = alloc_copy(&h->d_tab, &h->tab, 1);
+    if (e == hipSuccess) {
+        const std::vector<unsigned> fr = build_mxfrag(h->tab, n);
+        e = alloc_copy(&h->d_mxfrag, fr.data(), fr.size());
+    }
     if (e == hipSuccess) e = alloc_copy(&h->d_unodes, d->u_nodes, (size_t)(d->nu + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_vnodes, d->v_nodes, (size_t)(d->nv + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_wts, d->weights, (size_t)n);
@@ -575,7 +646,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
     void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts,   h->d_kwave, h->d_T,
-                    h->d_phases, h->d_bg,     h->d_bg32, h->d_tab};
+                    h->d_phases, h->d_bg,     h->d_bg32, h->d_tab, h->d_mxfrag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -792,7 +863,8 @@ int vl_set_tuning(vl_geom_t h, int key, int value) {
         if (value < 1) return fail(VL_E_ARG, "xseg must be >= 1");
         h->xseg = std::min(value, h->nx);
         h->pencils = h->nu * h->nchunk * ((h->nx + h->xseg - 1) / h->xseg);
-        h->pencils4 = h->tiles_per_ex4 * ((h->nx + h->xseg - 1) / h->xseg);
+        h->xseg4 = h->xseg;
+        h->pencils4 = h->tiles_per_ex4 * ((h->nx + h->xseg4 - 1) / h->xseg4);
         return VL_OK;
     }
     if (key == VL_TUNE_ROW_SPLIT) {
