@@ -11,5 +11,5 @@ from .basis import Basis1D, Basis3D, lobatto  # noqa: F401
 from .grid import Grid1D, Grid3D, Distribution, seeded_phases  # noqa: F401
 from .fluxes import DGFlux  # noqa: F401
 from .elliptic import Elliptic  # noqa: F401
-from .timestep import Stepper, get_max_speeds  # noqa: F401
+from .timestep import Stepper, StepGraph, get_max_speeds  # noqa: F401
 from .workload import Run, CONFIGS  # noqa: F401

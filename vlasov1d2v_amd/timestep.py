@@ -8,6 +8,7 @@ cross-checking.  With a fixed ``dt`` the loop never synchronises and can be
 captured in a HIP graph; with ``dt=None`` it follows the reference's CFL rule
 (main/timestep.py:184,231-242), which reads max|acc E| back every step.
 """
+import gc
 import time as _time
 
 import numpy as np
@@ -188,11 +189,13 @@ class Stepper:
 
     # -- time loop -------------------------------------------------------------
     def main_loop(self, distribution, basis, elliptic, grids, dg_flux, refs=None, save_file=None,
-                  steps=None):
+                  steps=None, graph_steps=0):
         """Step until ``final_time`` (or exactly ``steps`` steps) and record the
         field-energy series as the reference does (main/timestep.py:81-139).
 
-        With a fixed dt the energies stay on the GPU until the loop ends.
+        With a fixed dt the energies stay on the GPU until the loop ends, and
+        with ``graph_steps`` > 0 the loop replays a HIP graph of that many
+        steps (after one eager step; the remainder runs eagerly).
         """
         import torch
         t0 = _time.time()
@@ -214,13 +217,68 @@ class Stepper:
             energies = torch.empty(steps + 1, dtype=torch.float64, device=dev)
             energies[0] = e0
             self.resident_begin(distribution, grids, dg_flux, elliptic)
-            for k in range(steps):
-                self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
-                                      energy_out=energies[k + 1:k + 2])
-                self.time += self.dt
-                self.steps_counter += 1
-                self.time_array = np.append(self.time_array, self.time)
+            k = 0
+            graph = None
+            while k < steps:
+                if graph_steps > 0 and k >= 1 and steps - k >= graph_steps:
+                    if graph is None:
+                        graph = StepGraph(self, distribution, basis, elliptic, grids, dg_flux, graph_steps, refs)
+                    graph.replay()
+                    energies[k + 1:k + 1 + graph_steps].copy_(graph.energies)
+                    n_done = graph_steps
+                else:
+                    self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
+                                          energy_out=energies[k + 1:k + 2])
+                    n_done = 1
+                for _ in range(n_done):
+                    self.time += self.dt
+                    self.steps_counter += 1
+                    self.time_array = np.append(self.time_array, self.time)
+                k += n_done
             self.resident_end(distribution)
             torch.cuda.synchronize(dev)
             self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
         self.elapsed = _time.time() - t0
+
+
+class StepGraph:
+    """``steps`` fixed-dt SSP-RK3 steps captured once in a HIP graph.
+
+    The step sequence launches kernels only (no host synchronisation, no
+    allocation, no memset), so it is captured as is on a side stream of
+    default priority.  The first captured step reuses the moment partials the
+    previous step left in the workspace, so the state must have been stepped
+    once (eagerly) before capture.  ``replay()`` advances the distribution by
+    ``steps`` steps and leaves the energies of those steps in ``energies``.
+    """
+
+    def __init__(self, stepper, distribution, basis, elliptic, grids, dg_flux, steps, refs=None):
+        import torch
+        if stepper.fixed_dt is None:
+            raise ValueError("a captured step needs a fixed dt")
+        if stepper._rho_token is None:
+            raise RuntimeError("step once eagerly before capturing (the first captured step reuses "
+                               "the moment partials of the previous step)")
+        dev = distribution.arr.device
+        self.steps = int(steps)
+        self.energies = torch.zeros(self.steps, dtype=torch.float64, device=dev)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        self.graph = torch.cuda.CUDAGraph()
+        # no garbage collection during capture: a collected geometry handle
+        # would free device memory in the middle of it
+        gc.collect()
+        gc_was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            with torch.cuda.graph(self.graph, stream=side, capture_error_mode="thread_local"):
+                for k in range(self.steps):
+                    stepper.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
+                                             energy_out=self.energies[k:k + 1])
+        finally:
+            if gc_was_enabled:
+                gc.enable()
+        torch.cuda.current_stream(dev).wait_stream(side)
+
+    def replay(self):
+        self.graph.replay()

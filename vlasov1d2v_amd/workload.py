@@ -95,6 +95,12 @@ class Run:
     def resident_end(self):
         self.stepper.resident_end(self.dist)
 
-    def run(self, steps):
-        self.stepper.main_loop(self.dist, self.basis, self.elliptic, self.grids, self.flux, steps=steps)
+    def run(self, steps, graph_steps=0):
+        self.stepper.main_loop(self.dist, self.basis, self.elliptic, self.grids, self.flux, steps=steps,
+                               graph_steps=graph_steps)
         return self.stepper
+
+    def capture(self, steps):
+        """A HIP graph of ``steps`` steps of this run (see StepGraph)."""
+        from .timestep import StepGraph
+        return StepGraph(self.stepper, self.dist, self.basis, self.elliptic, self.grids, self.flux, steps)
