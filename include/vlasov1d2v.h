@@ -109,8 +109,14 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
  * layout on the matrix cores, 0 on the vector ALUs.
  * VL_TUNE_ROW_SPLIT: 1 (default) makes vl_resident_layout report the
  * row-split layout for BF16/FP16 storage with n = 8 or 9 (while
- * VL_TUNE_MFMA is 1); 0 keeps the element-major layout. */
-enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2, VL_TUNE_ROW_SPLIT = 3 };
+ * VL_TUNE_MFMA is 1); 0 keeps the element-major layout.
+ * VL_TUNE_ROUNDING: rounding of the 16-bit stored values by the row-split
+ * stage kernel: 1 (default) stochastic -- unbiased, so that stage updates
+ * smaller than one storage ulp are not lost (round-to-nearest damps the
+ * field at small dt); the random bits are a hash of a per-stage seed kept in
+ * the workspace and the value's index, so runs are reproducible; 0
+ * round-to-nearest-even (as the other 16-bit kernels). */
+enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2, VL_TUNE_ROW_SPLIT = 3, VL_TUNE_ROUNDING = 4 };
 int vl_set_tuning(vl_geom_t h, int key, int value);
 int vl_get_tuning(vl_geom_t h, int key, int *value);
 

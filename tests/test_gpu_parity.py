@@ -159,7 +159,9 @@ def _three_resident_steps(storage, n, nx, nu, nv, mfma, layout):
     from vlasov1d2v_amd import Run, _lib
     from vlasov1d2v_amd.grid import STORAGE_CODES
     r = Run(n, nx, nu, nv, dt=0.004, storage=storage)
-    r.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, mfma)
+    g = r.grids.geometry(STORAGE_CODES[storage])
+    g.set_tuning(_lib.VL_TUNE_MFMA, mfma)
+    g.set_tuning(_lib.VL_TUNE_ROUNDING, 0)    # round to nearest, as the vector-ALU kernel
     r.resident_begin(layout=layout)
     for _ in range(3):
         r.step()

@@ -199,6 +199,11 @@ struct StageArgs {
     // matrix-core operator fragments of the row-split kernel: [format][lane][4]
     // (format 0 BF16, 1 FP16; words hi.x, hi.y, lo.x, lo.y)
     const unsigned *mxfrag;
+    // rounding of the 16-bit stored values (row-split kernel): 0 nearest-even,
+    // 1 stochastic with random bits hashed from (*seed, value index); *seed
+    // is a workspace word advanced on the device before every stage
+    const unsigned *seed;
+    int rounding;
     double scale, inv_scale;       // power of two
 };
 
@@ -244,6 +249,17 @@ struct DD {
         return hi + lo;
     }
 };
+
+// 32-bit integer hash (xorshift-multiply "lowbias32" constants, C. Wellons);
+// counter-based random bits for stochastic rounding.
+__host__ __device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
 
 __device__ __forceinline__ double mul_rn(double a, double b) {
 #pragma clang fp contract(off)

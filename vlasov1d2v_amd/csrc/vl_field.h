@@ -67,9 +67,13 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
 // rho[(ex-1)*n + ix] = sum over the tiles of x-element ex of the moment
 // partials, in a fixed order: thread t owns tiles t, t+256, ... (double-double),
 // then a fixed tree over the 256 threads.  One workgroup per x element.
+// When `seed` is given, workgroup 0 also advances it by one (the stage kernel
+// that follows reads it as the seed of its stochastic rounding).
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__restrict__ partials,
-                                                            int tiles_per_ex, double *__restrict__ rho) {
+                                                            int tiles_per_ex, double *__restrict__ rho,
+                                                            unsigned *__restrict__ seed) {
+    if (seed && blockIdx.x == 0 && threadIdx.x == 0) seed[0] += 1u;
     __shared__ double sh[N][kBlock];
     __shared__ double sl[N][kBlock];
     const int ex0 = blockIdx.x;

@@ -106,6 +106,16 @@ template <> struct RsPack<2> {              // BF16
     }
     __device__ static float lo(unsigned w) { return __builtin_bit_cast(float, w << 16); }
     __device__ static float hi(unsigned w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+    // stochastic rounding: add 16 random bits below the BF16 mantissa and
+    // truncate (rounds up with probability equal to the discarded fraction)
+    __device__ static unsigned sr2(float a, float b, unsigned r) {
+        const unsigned ua = __builtin_bit_cast(unsigned, a) + (r & 0xffffu);
+        const unsigned ub = __builtin_bit_cast(unsigned, b) + (r >> 16);
+        return __builtin_amdgcn_perm(ub, ua, 0x07060302u);
+    }
+    __device__ static unsigned sr1(float a, unsigned r) {
+        return (__builtin_bit_cast(unsigned, a) + (r & 0xffffu)) >> 16;
+    }
 };
 template <> struct RsPack<3> {              // FP16
     __device__ static unsigned exact2(float a, float b) {
@@ -113,6 +123,17 @@ template <> struct RsPack<3> {              // FP16
     }
     __device__ static float lo(unsigned w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
     __device__ static float hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+    // stochastic rounding: uniform noise of one FP16 spacing (the spacing at
+    // a's binade, 2^-24 in the subnormal range), then round to nearest
+    __device__ static float dither(float a, unsigned r16) {
+        const unsigned e = (__builtin_bit_cast(unsigned, a) >> 23) & 0xffu;
+        const float ulp = __builtin_bit_cast(float, ((e < 113u ? 113u : e) - 10u) << 23);
+        return a + ((float)r16 * (1.0f / 65536.0f) - 0.5f) * ulp;
+    }
+    __device__ static unsigned sr2(float a, float b, unsigned r) {
+        return MxFmt<3>::enc2(dither(a, r & 0xffffu), dither(b, r >> 16));
+    }
+    __device__ static unsigned sr1(float a, unsigned r) { return MxFmt<3>::enc(dither(a, r & 0xffffu)); }
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -199,6 +220,8 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     unsigned vsel = kSelKeep;
     unsigned xposm = 0u;               // bit su: u(eu, su) >= 0 (x-line upwinding), wave-uniform
     const float jua = (float)(s.ju * s.acc);
+    const bool sround = s.rounding != 0;
+    const unsigned seedh = sround ? hash32(*s.seed * 0x9e3779b9u + 0x7f4a7c15u) : 0u;
     if (active) {
         const int64_t W = (int64_t)vrow * N;
         const int64_t tbl = (int64_t)(s.nu + 2) * N * W;
@@ -415,6 +438,8 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                     const unsigned nw[4] = {fnr.x, fnr.y, fnr.z, fnr.w};
                     ConstOpTables &tab = *(ConstOpTables *)tabp;
                     // pairs of values: combine, round, moment of the stored pair
+                    // random words of this row (stochastic rounding): hash of (value row, pair)
+                    const unsigned rowgid = ((unsigned)(((ex * (s.nu + 2) + eu) * vrow + ev) * ROWS + r)) << 3;
                     auto pair = [&](float l0, float l1, unsigned fiw, unsigned fnw, float &m, int j) {
                         float o0, o1;
                         if (MODE == kModeStage1) {
@@ -424,7 +449,8 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                             o0 = (c0 * PK::lo(fnw) + c1 * PK::lo(fiw)) + c2 * l0;
                             o1 = (c0 * PK::hi(fnw) + c1 * PK::hi(fiw)) + c2 * l1;
                         }
-                        const unsigned wv2 = F::enc2(o0, o1);
+                        const unsigned wv2 = sround ? PK::sr2(o0, o1, hash32((rowgid | (unsigned)(j >> 1)) ^ seedh))
+                                                    : F::enc2(o0, o1);
                         m += tab.wv[j] * PK::lo(wv2);
                         m += tab.wv[j + 1] * PK::hi(wv2);
                         return wv2;
@@ -448,7 +474,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                         const float fi8 = PK::lo(br[4]);
                         const float fn8 = (r & 1) ? PK::hi(fnt) : PK::lo(fnt);
                         const float o8 = MODE == kModeStage1 ? fi8 + c2 * xp[8] : (c0 * fn8 + c1 * fi8) + c2 * xp[8];
-                        const unsigned t8 = F::enc(o8);
+                        const unsigned t8 = sround ? PK::sr1(o8, hash32((rowgid | 4u) ^ seedh)) : F::enc(o8);
                         __builtin_amdgcn_raw_buffer_store_b16((uint16_t)t8, rout, (int)(bo * 2) + (8 * ROWS + r) * 2,
                                                               0, 0);
                         m += tab.wv[8] * PK::lo(t8);

@@ -32,6 +32,7 @@ struct vl_geom {
     int row_split;         // 1: the 16-bit forms with n = 8, 9 prefer the row-split layout
     int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: 4 v-elements per tile
     int xseg4;             // row-split stage kernel: x elements per pencil
+    int rounding;          // 16-bit forms, row-split kernel: 0 nearest-even, 1 stochastic
     int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
@@ -112,6 +113,7 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
     s.bg = h->d_bg;
     s.bg32 = h->d_bg32;
     s.mxfrag = h->d_mxfrag;
+    s.rounding = h->rounding;
     s.scale = h->scale;
     s.inv_scale = h->inv_scale;
     return s;
@@ -288,10 +290,13 @@ double *ws_mxpart(const vl_geom *h, void *ws) {
 unsigned int *ws_ticket(const vl_geom *h, void *ws) {
     return reinterpret_cast<unsigned int *>(static_cast<char *>(ws) + h->off_ticket);
 }
+// rounding seed: the word after the ticket (zeroed with it by vl_workspace_init)
+unsigned int *ws_seed(const vl_geom *h, void *ws) { return ws_ticket(h, ws) + 1; }
 
 template <int N>
-hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, int tpe, hipStream_t st) {
-    hipLaunchKernelGGL((k_reduce_partials<N>), dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws), tpe, rho);
+hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, int tpe, unsigned *seed, hipStream_t st) {
+    hipLaunchKernelGGL((k_reduce_partials<N>), dim3(h->nx), dim3(kBlock), 0, st, ws_partials(h, ws), tpe, rho,
+                       seed);
     return hipGetLastError();
 }
 
@@ -300,18 +305,19 @@ int tiles_per_ex_of(const vl_geom *h, int layout) {
     return layout == kRowSplitLayout ? h->tiles_per_ex4 * 4 : h->tiles_per_ex;   // row-split: one per element
 }
 
-hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st, int layout = kRefLayout) {
+hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st, int layout = kRefLayout,
+                         unsigned *seed = nullptr) {
     const int tpe = tiles_per_ex_of(h, layout);
     switch (h->n) {
-        case 2: return launch_reduce_n<2>(h, ws, rho, tpe, st);
-        case 3: return launch_reduce_n<3>(h, ws, rho, tpe, st);
-        case 4: return launch_reduce_n<4>(h, ws, rho, tpe, st);
-        case 5: return launch_reduce_n<5>(h, ws, rho, tpe, st);
-        case 6: return launch_reduce_n<6>(h, ws, rho, tpe, st);
-        case 7: return launch_reduce_n<7>(h, ws, rho, tpe, st);
-        case 8: return launch_reduce_n<8>(h, ws, rho, tpe, st);
-        case 9: return launch_reduce_n<9>(h, ws, rho, tpe, st);
-        case 10: return launch_reduce_n<10>(h, ws, rho, tpe, st);
+        case 2: return launch_reduce_n<2>(h, ws, rho, tpe, seed, st);
+        case 3: return launch_reduce_n<3>(h, ws, rho, tpe, seed, st);
+        case 4: return launch_reduce_n<4>(h, ws, rho, tpe, seed, st);
+        case 5: return launch_reduce_n<5>(h, ws, rho, tpe, seed, st);
+        case 6: return launch_reduce_n<6>(h, ws, rho, tpe, seed, st);
+        case 7: return launch_reduce_n<7>(h, ws, rho, tpe, seed, st);
+        case 8: return launch_reduce_n<8>(h, ws, rho, tpe, seed, st);
+        case 9: return launch_reduce_n<9>(h, ws, rho, tpe, seed, st);
+        case 10: return launch_reduce_n<10>(h, ws, rho, tpe, seed, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -335,7 +341,7 @@ hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho,
 // partials -> rho (workspace) -> E (+phi, energy)
 hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *phi, double *energy,
                                hipStream_t st, int layout) {
-    hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st, layout);
+    hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st, layout, ws_seed(h, ws));
     if (e != hipSuccess) return e;
     return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st);
 }
@@ -581,6 +587,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     }
     // row-split kernels: tiles of 4 v-elements (same x-segments)
     h->row_split = 1;
+    h->rounding = 1;
     h->nchunk4 = (d->nv + 3) / 4;
     h->tiles_per_ex4 = d->nu * h->nchunk4;
     h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
@@ -779,6 +786,7 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.periodic_x = 1;
     s.write_partials = 1;
     s.partials = ws_partials(h, ws);
+    s.seed = ws_seed(h, ws);
     if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, LAY, f, s, st));
     // stage 1: f1 = f + dt L(f)                       (main/timestep.py:153-163)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st, LAY));
@@ -872,6 +880,11 @@ int vl_set_tuning(vl_geom_t h, int key, int value) {
         h->row_split = value;
         return VL_OK;
     }
+    if (key == VL_TUNE_ROUNDING) {
+        if (value != 0 && value != 1) return fail(VL_E_ARG, "rounding must be 0 or 1");
+        h->rounding = value;
+        return VL_OK;
+    }
     if (key == VL_TUNE_MFMA) {
         if (value != 0 && value != 1) return fail(VL_E_ARG, "mfma must be 0 or 1");
         h->mfma = value;
@@ -893,6 +906,10 @@ int vl_get_tuning(vl_geom_t h, int key, int *value) {
     }
     if (key == VL_TUNE_ROW_SPLIT) {
         *value = h->row_split;
+        return VL_OK;
+    }
+    if (key == VL_TUNE_ROUNDING) {
+        *value = h->rounding;
         return VL_OK;
     }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
