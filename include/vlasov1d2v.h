@@ -216,6 +216,52 @@ int vl_set_stage_events(vl_geom_t h, void **events);
  * (nu+2)*n*(nv+2)*n values.  Only used when storage != VL_F64. */
 int vl_set_background(vl_geom_t h, const double *f0_host);
 
+/* ---- Ensembles (no counterpart in the reference, which steps one run per
+ * process through main/main.py:165 -> Stepper.main_loop).  Many independent
+ * runs -- each its own geometry (n, grid, physics), state and workspace --
+ * step together: every RK stage of every run is one launch per kernel kind
+ * (ghost swap, moment reduction, forward DFT, inverse DFT, stage), so one
+ * ensemble step is 13 launches whatever the number of runs.  Each run
+ * computes what vl_rk3_step with VL_STEP_ELEMENT_MAJOR computes for it
+ * (main/timestep.py:141-187 per run), the same statements up to the
+ * compiler's multiply-add fusion (rounding level); the potential phi is not
+ * formed.  Storage FP64 or FP32 (one for all
+ * runs), n = 6..9.  The geometries must outlive the ensemble. */
+typedef struct vl_ensemble *vl_ensemble_t;
+int vl_ensemble_create(const vl_geom_t *geoms, int count, vl_ensemble_t *out);
+int vl_ensemble_destroy(vl_ensemble_t e);
+
+/* Buffers and time step of run `run`: f, f1, f2 in the element-major layout
+ * (16-byte aligned, vl_state_elems_layout values each), E (nx+2)*n, the run's
+ * workspace (vl_workspace_bytes, initialised), and its fixed dt. */
+int vl_ensemble_bind(vl_ensemble_t e, int run, void *f, void *f1, void *f2, double *E, void *workspace,
+                     double dt);
+
+/* Schedule of the ensemble stage kernel: about `work_per_item` DOF per work
+ * item (one pencil of one run; sets each run's x-segment length), `grid`
+ * workgroups of the persistent launch (0: one workgroup per item), item
+ * `order` 0 = longest first (a run's pencils stay consecutive), 1 = run by
+ * run.  get: also the number of work items. */
+int vl_ensemble_set_schedule(vl_ensemble_t e, int work_per_item, int grid, int order);
+int vl_ensemble_get_schedule(vl_ensemble_t e, int *work_per_item, int *grid, int *order, int *items);
+
+/* Moment partials of every run's f: call once before the first step of a
+ * loop (a step leaves those of its result for the next one). */
+int vl_ensemble_moments(vl_ensemble_t e, void *stream);
+
+/* One SSP-RK3 step of every run.  energy_out (device, may be NULL): count
+ * values, run r's field energy of its stage-3 field (as vl_rk3_step).  The
+ * first call after vl_ensemble_bind / set_schedule uploads the run table
+ * (synchronous): make it outside a graph capture; later calls only launch. */
+int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream);
+
+/* Autotuner: times the stage-2 launch (reps times each) over x-segment work
+ * targets {4500 .. 72000 DOF}, persistent grids {one per item, 3, 6, 12 per
+ * CU} and both item orders; keeps the fastest schedule, then restores the
+ * moment partials of f.  table (may be NULL): up to max_rows rows of
+ * (work, grid, order, ms); *rows = rows measured.  Synchronises the stream. */
+int vl_ensemble_autotune(vl_ensemble_t e, int reps, float *table, int max_rows, int *rows, void *stream);
+
 /* Description of the last failure on this thread (static storage). */
 const char *vl_last_error_string(void);
 

@@ -113,3 +113,34 @@ def test_ulp_distance_helper():
         pos, neg = tiny.view(dt), (-tiny.view(dt))
         assert ulp_distance(pos, neg)[0] == 2
         assert ulp_distance(torch.tensor([0.0], dtype=dt), torch.tensor([-0.0], dtype=dt))[0] == 0
+
+
+def test_ensemble_members_follow_the_workload_ranges():
+    from vlasov1d2v_amd import draw_members
+    a = draw_members(48, seed=0)
+    assert a == draw_members(48, seed=0) and a != draw_members(48, seed=1)
+    assert len(a) == 48 and len({m.seed for m in a}) == 48
+    for m in a:
+        assert m.nx in (16, 32, 48, 64) and 24 <= m.nu <= 48 and m.nu == m.nv
+        assert 6 <= m.n <= 9 and 1 <= m.ring_j <= 8
+        assert 0.05 <= m.k0 <= 0.2 and 0.05 <= m.b_field <= 0.2
+    # every node count and x resolution occurs in the 48 draws
+    assert {m.n for m in a} == {6, 7, 8, 9} and {m.nx for m in a} == {16, 32, 48, 64}
+
+
+def test_ensemble_entry_points_validate_without_a_device():
+    from vlasov1d2v_amd import _lib, Ensemble
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.vl_ensemble_create(None, 1, ctypes.byref(h)) == -2
+    handles = (ctypes.c_void_p * 1)(None)
+    assert lib.vl_ensemble_create(handles, 0, ctypes.byref(h)) == -1
+    assert lib.vl_ensemble_create(handles, 1, ctypes.byref(h)) == -1      # no geometry: no storage
+    assert lib.vl_ensemble_bind(None, 0, None, None, None, None, None, 0.01) == -4
+    assert lib.vl_ensemble_rk3_step(None, None, None) == -4
+    assert lib.vl_ensemble_set_schedule(None, 1, 0, 0) == -4
+    assert lib.vl_ensemble_destroy(None) == 0
+    with pytest.raises(ValueError):
+        Ensemble([], storage="bf16")

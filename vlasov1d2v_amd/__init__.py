@@ -13,3 +13,4 @@ from .fluxes import DGFlux  # noqa: F401
 from .elliptic import Elliptic  # noqa: F401
 from .timestep import Stepper, StepGraph, get_max_speeds  # noqa: F401
 from .workload import Run, CONFIGS  # noqa: F401
+from .ensemble import Ensemble, Member, draw_members  # noqa: F401

@@ -21,6 +21,8 @@ EXPORTS = (
     "vl_swap_ghosts", "vl_rk3_step", "vl_set_background", "vl_set_physics", "vl_set_stage_events",
     "vl_workspace_init", "vl_set_tuning", "vl_get_tuning", "vl_state_elems_layout",
     "vl_convert_layout", "vl_resident_layout", "vl_last_error_string",
+    "vl_ensemble_create", "vl_ensemble_destroy", "vl_ensemble_bind", "vl_ensemble_set_schedule",
+    "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -83,6 +85,15 @@ def load(path=LIB_PATH):
         "vl_state_elems_layout": [vp, i32, ctypes.POINTER(sz)],
         "vl_convert_layout": [vp, vp, i32, vp, i32, vp],
         "vl_resident_layout": [vp, ctypes.POINTER(i32)],
+        "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
+        "vl_ensemble_destroy": [vp],
+        "vl_ensemble_bind": [vp, i32, vp, vp, vp, vp, vp, f64],
+        "vl_ensemble_set_schedule": [vp, i32, i32, i32],
+        "vl_ensemble_get_schedule": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                     ctypes.POINTER(i32)],
+        "vl_ensemble_moments": [vp, vp],
+        "vl_ensemble_rk3_step": [vp, vp, vp],
+        "vl_ensemble_autotune": [vp, i32, ctypes.POINTER(ctypes.c_float), i32, ctypes.POINTER(i32), vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

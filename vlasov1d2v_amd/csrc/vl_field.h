@@ -69,14 +69,14 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
 // then a fixed tree over the 256 threads.  One workgroup per x element.
 // When `seed` is given, workgroup 0 also advances it by one (the stage kernel
 // that follows reads it as the seed of its stochastic rounding).
+// Body for x element ex0 (sh, sl: N * kBlock doubles of LDS each).
 template <int N>
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__restrict__ partials,
-                                                            int tiles_per_ex, double *__restrict__ rho,
-                                                            unsigned *__restrict__ seed) {
-    if (seed && blockIdx.x == 0 && threadIdx.x == 0) seed[0] += 1u;
-    __shared__ double sh[N][kBlock];
-    __shared__ double sl[N][kBlock];
-    const int ex0 = blockIdx.x;
+__device__ __forceinline__ void reduce_partials_body(const double2 *__restrict__ partials, int tiles_per_ex,
+                                                     double *__restrict__ rho, unsigned *__restrict__ seed,
+                                                     int ex0, double *sh_, double *sl_) {
+    if (seed && ex0 == 0 && threadIdx.x == 0) seed[0] += 1u;
+    double (*sh)[kBlock] = reinterpret_cast<double (*)[kBlock]>(sh_);
+    double (*sl)[kBlock] = reinterpret_cast<double (*)[kBlock]>(sl_);
     const int tid = threadIdx.x;
     DD acc[N];
     const double2 *base = partials + (int64_t)ex0 * tiles_per_ex * N;
@@ -116,6 +116,15 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__res
     if (tid < N) rho[ex0 * N + tid] = sh[tid][0] + sl[tid][0];
 }
 
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double2 *__restrict__ partials,
+                                                            int tiles_per_ex, double *__restrict__ rho,
+                                                            unsigned *__restrict__ seed) {
+    __shared__ double sh[N * kBlock];
+    __shared__ double sl[N * kBlock];
+    reduce_partials_body<N>(partials, tiles_per_ex, rho, seed, blockIdx.x, sh, sl);
+}
+
 struct FieldArgs {
     int nx, n, M;
     int from_rho;            // 1: charge density = mean(rho) - rho; 0: given
@@ -129,15 +138,13 @@ struct FieldArgs {
 //          identically: same fixed-order double-double sum)
 //   c_m  = (sum_k (poisson_coef*cd_k) T[k][m]) * dx / length
 //   Ehat = -c/(i k), phihat = -c/k^2, k = 0 -> 0     (main/elliptic.py:124-128)
-__global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict__ rho_or_cd,
-                                                        const double2 *__restrict__ T,   // [k][m]
-                                                        const double *__restrict__ kwave,
-                                                        double2 *__restrict__ coef,      // [2][M]
-                                                        FieldArgs a) {
-    extern __shared__ double dyn[];
-    double *rhs = dyn;                                  // K values
-    __shared__ double sh[kBlock];
-    __shared__ double sl[kBlock];
+// Body for mode group bx (rhs: nx*n doubles, sh, sl: kBlock doubles of LDS).
+__device__ __forceinline__ void dft_forward_body(const double *__restrict__ rho_or_cd,
+                                                 const double2 *__restrict__ T,   // [k][m]
+                                                 const double *__restrict__ kwave,
+                                                 double2 *__restrict__ coef,      // [2][M]
+                                                 const FieldArgs &a, int bx, double *rhs, double *sh,
+                                                 double *sl) {
     const int tid = threadIdx.x;
     const int K = a.nx * a.n;
     double mean = 0.0;
@@ -155,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict
     constexpr int G = kBlock / kDftModes;
     const int mm = tid % kDftModes;
     const int g = tid / kDftModes;
-    const int m = blockIdx.x * kDftModes + mm;
+    const int m = bx * kDftModes + mm;
     double re = 0.0, im = 0.0;
     if (m < a.M) {
         // batches of 8 independent table loads, then the multiply-adds in k order
@@ -204,19 +211,36 @@ __global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict__ rho_or_cd,
+                                                        const double2 *__restrict__ T,
+                                                        const double *__restrict__ kwave,
+                                                        double2 *__restrict__ coef, FieldArgs a) {
+    extern __shared__ double dyn[];                   // nx*n values
+    __shared__ double sh[kBlock];
+    __shared__ double sl[kBlock];
+    dft_forward_body(rho_or_cd, T, kwave, coef, a, blockIdx.x, dyn, sh, sl);
+}
+
 // Inverse sums at the nodes: E[1:-1] = Re sum_m Ehat_m phases[m][k] (and phi),
 // periodic ghosts, and -- when energy/maxabs are requested -- the field energy
 // 0.5 sum w E^2 and max|acc E|, reduced in a fixed order by the last workgroup
 // to finish (ticket counter in the workspace; it returns to zero).
-__global__ __launch_bounds__(kBlock) void k_dft_inverse(
+// LDS of the inverse sums besides the 2*M coefficients
+struct InvLds {
+    double pe[kBlock], pp[kBlock];
+    double sh[kInvNodes], sl[kInvNodes], smx[kInvNodes];
+    int is_last;
+};
+
+// Body for node group bx of nblocks (cdyn: 2*M complex of LDS).
+__device__ __forceinline__ void dft_inverse_body(
     const double2 *__restrict__ coef, const double2 *__restrict__ phases,   // [2][M], [m][k]
     const double *__restrict__ wts, double *__restrict__ E, double *__restrict__ phi,
     double2 *__restrict__ en_part, double *__restrict__ mx_part, unsigned int *__restrict__ ticket,
-    double *__restrict__ energy, double *__restrict__ maxabs, FieldArgs a) {
-    extern __shared__ double2 cdyn[];                 // 2*M coefficients
-    __shared__ double pe[kBlock], pp[kBlock];
-    __shared__ double sh[kInvNodes], sl[kInvNodes], smx[kInvNodes];
-    __shared__ int is_last;
+    double *__restrict__ energy, double *__restrict__ maxabs, const FieldArgs &a, int bx, int nblocks,
+    double2 *cdyn, InvLds &ls) {
+    double *pe = ls.pe, *pp = ls.pp, *sh = ls.sh, *sl = ls.sl, *smx = ls.smx;
+    int &is_last = ls.is_last;
     const int tid = threadIdx.x;
     const int K = a.nx * a.n;
     const int M = a.M;
@@ -224,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_dft_inverse(
     __syncthreads();
     const int kk = tid % kInvNodes;
     const int g = tid / kInvNodes;
-    const int k = blockIdx.x * kInvNodes + kk;
+    const int k = bx * kInvNodes + kk;
     double e = 0.0, p = 0.0;
     if (k < K) {
         // batches of 8 independent phase loads, then the sums in m order
@@ -291,11 +315,11 @@ __global__ __launch_bounds__(kBlock) void k_dft_inverse(
             s.add(d);
             mx = fmax(mx, smx[i]);
         }
-        en_part[blockIdx.x] = make_double2(s.hi, s.lo);
-        mx_part[blockIdx.x] = mx;
+        en_part[bx] = make_double2(s.hi, s.lo);
+        mx_part[bx] = mx;
         __threadfence();
         const unsigned int prev = atomicAdd(ticket, 1u);
-        is_last = (prev == gridDim.x - 1);
+        is_last = (prev == (unsigned)nblocks - 1);
     }
     __syncthreads();
     if (!is_last) return;
@@ -305,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_dft_inverse(
         double mx = 0.0;
         const volatile double *vp = reinterpret_cast<const volatile double *>(en_part);
         const volatile double *vm = mx_part;
-        for (int b = 0; b < (int)gridDim.x; ++b) {
+        for (int b = 0; b < nblocks; ++b) {
             DD d;
             d.hi = vp[2 * b];
             d.lo = vp[2 * b + 1];
@@ -316,6 +340,17 @@ __global__ __launch_bounds__(kBlock) void k_dft_inverse(
         if (maxabs) maxabs[0] = mx;
         atomicExch(ticket, 0u);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dft_inverse(
+    const double2 *__restrict__ coef, const double2 *__restrict__ phases, const double *__restrict__ wts,
+    double *__restrict__ E, double *__restrict__ phi, double2 *__restrict__ en_part,
+    double *__restrict__ mx_part, unsigned int *__restrict__ ticket, double *__restrict__ energy,
+    double *__restrict__ maxabs, FieldArgs a) {
+    extern __shared__ double2 cdyn[];                 // 2*M coefficients
+    __shared__ InvLds ls;
+    dft_inverse_body(coef, phases, wts, E, phi, en_part, mx_part, ticket, energy, maxabs, a, blockIdx.x,
+                     gridDim.x, cdyn, ls);
 }
 
 // energy[0] = 0.5 * sum_interior w_j E^2 (main/elliptic.py:150-151)
@@ -353,10 +388,10 @@ __global__ void k_zero_words(unsigned int *__restrict__ p, int count) {
 // Periodic x-ghost copy, in place: plane 0 <- plane nx, plane nx+1 <- plane 1.
 // Planes are contiguous (the x element is the outermost axis).
 template <typename W>
-__global__ __launch_bounds__(kBlock) void k_swap_ghosts(W *__restrict__ f, int64_t plane_words, int nx) {
+__device__ __forceinline__ void swap_ghosts_body(W *__restrict__ f, int64_t plane_words, int nx, int bx,
+                                                 int nblocks) {
     const int64_t total = 2 * plane_words;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * kBlock) {
+    for (int64_t i = (int64_t)bx * kBlock + threadIdx.x; i < total; i += (int64_t)nblocks * kBlock) {
         const int side = i >= plane_words;
         const int64_t o = i - side * plane_words;
         if (side == 0)
@@ -364,6 +399,11 @@ __global__ __launch_bounds__(kBlock) void k_swap_ghosts(W *__restrict__ f, int64
         else
             f[(int64_t)(nx + 1) * plane_words + o] = f[plane_words + o];
     }
+}
+
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_swap_ghosts(W *__restrict__ f, int64_t plane_words, int nx) {
+    swap_ghosts_body<W>(f, plane_words, nx, blockIdx.x, gridDim.x);
 }
 
 }  // namespace vl

@@ -128,8 +128,8 @@ struct Pencil {
     int chunk, eu, ex_begin, ex_end;
 };
 
-__device__ __forceinline__ Pencil decode_pencil(const StageArgs &s) {
-    const int p = xcd_remap(blockIdx.x, gridDim.x);
+// Logical pencil p -> (v-chunk, eu, x-segment) of one run.
+__device__ __forceinline__ Pencil decode_pencil(const StageArgs &s, int p) {
     Pencil c;
     c.chunk = p % s.nchunk;
     const int t2 = p / s.nchunk;
@@ -139,6 +139,31 @@ __device__ __forceinline__ Pencil decode_pencil(const StageArgs &s) {
     c.ex_end = min(s.nx, c.ex_begin + s.xseg - 1);
     return c;
 }
+
+// LDS arrays of one stage workgroup.  The single-run kernel declares them as
+// separate __shared__ arrays; the ensemble kernel carves them out of one
+// buffer (byte offsets below, FP64 arrays first).
+template <int N, int S>
+struct StageLds {
+    using C = typename Store<S>::C;
+    static constexpr bool BG = Store<S>::kDelta;
+    static constexpr int P = TileShape<N>::P, PP = TileShape<N>::PP;
+    static constexpr int n_ft = N * N * PP;   // tile rows (ix,iu) x v-nodes with halo
+    static constexpr int n_x = N * P;         // left / right x halo [iu][q]
+    static constexpr int n_uh = 2 * N * P;    // u halo [side][ix][q]
+    static constexpr int n_acc = N * N * P;   // output accumulator rows (ix,iu) x q
+    static constexpr int n_bgs = BG ? 3 * N * P : 1;   // background slab [table][iu][q]
+    static constexpr int n_red = BG ? N * P : 1;       // moment scratch (the FP64 form reuses uh)
+    static constexpr size_t o_bgs = 0;
+    static constexpr size_t o_red = o_bgs + (size_t)n_bgs * 8;
+    static constexpr size_t o_ft = o_red + (size_t)n_red * 8;
+    static constexpr size_t o_xl = o_ft + (size_t)n_ft * sizeof(C);
+    static constexpr size_t o_xr = o_xl + (size_t)n_x * sizeof(C);
+    static constexpr size_t o_uh = o_xr + (size_t)n_x * sizeof(C);
+    static constexpr size_t o_acc = o_uh + (size_t)n_uh * sizeof(C);
+    static constexpr size_t bytes = o_acc + (size_t)n_acc * sizeof(C);
+    static_assert(BG || (o_uh % 8 == 0), "the FP64 moment scratch aliases uh");
+};
 
 template <int N, int MODE, int S, int L>
 __global__ __launch_bounds__(kBlock, 3) void k_stage(
@@ -150,510 +175,16 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
     const double *__restrict__ unodes, // (nu+2)*N
     const double *__restrict__ vnodes, // (nv+2)*N
     StageArgs s, const OpTables *__restrict__ tabp) {
-    using ST = Store<S>;
-    using T = typename ST::E;
-    using C = typename ST::C;
-    constexpr bool BG = ST::kDelta;
-    using TS = TileShape<N>;
-    constexpr int TV = TS::TV, P = TS::P, PP = TS::PP;
-    __shared__ C ft[N * N * PP];           // tile rows (ix,iu) x v-nodes with halo
-    __shared__ C xl[N * P];                // left x halo  [iu][q]
-    __shared__ C xr[N * P];                // right x halo [iu][q]
-    __shared__ C uh[2 * N * P];            // u halo [side][ix][q]
-    __shared__ C acc_l[N * N * P];         // output accumulator rows (ix,iu) x q
-    __shared__ double bgs[BG ? 3 * N * P : 1];       // background slab [table][iu][q]
-    __shared__ double red_own[BG ? N * P : 1];       // moment scratch (FP64 form reuses uh)
-    double *red = BG ? red_own : reinterpret_cast<double *>(uh);
-
-    const int tid = threadIdx.x;
-    const Pencil pc = decode_pencil(s);
-    const int eu = pc.eu;
-    const int v0 = 1 + pc.chunk * TV;                    // first v element of the tile
-    const int nvalid = min(TV, s.nv + 1 - v0);           // valid v elements
-    const int pvalid = nvalid * N;
-    const int gv0 = v0 * N;                              // first v node of the tile
-    // offsets inside one x element fit in 32 bits (checked at handle creation)
-    const int sx = (int)s.sx, su = (int)s.su, sue = (int)s.sue;
-    const int base_u = eu * sue;
-    // pencil-constant parts of the speeds (main/fluxes.py:110,120-121,143)
-    const int lx_iu = tid / P, lx_q = tid - (tid / P) * P;        // x-line / u-line owner
-    const int lv_r = tid / TV, lv_el = tid - (tid / TV) * TV;     // v-line owner
-    const bool lx_on = tid < N * P && lx_q < pvalid;
-    const bool lv_on = tid < N * N * TV && lv_el < nvalid;
-    const double ax = lx_on ? unodes[eu * N + lx_iu] : 0.0;
-    const double bv = lx_on ? s.bfield * vnodes[gv0 + lx_q] : 0.0;
-    const double av = lv_on ? s.acc * (-s.bfield * unodes[eu * N + (lv_r - (lv_r / N) * N)]) : 0.0;
-    double e_cur = lx_on ? E[pc.ex_begin * N + lx_iu] : 0.0;     // lx_iu is ix for the u-line
-
-    auto xmap = [&](int e) {
-        if (s.periodic_x) {
-            if (e == 0) return s.nx;
-            if (e == s.nx + 1) return 1;
-        }
-        return e;
-    };
-
-    constexpr int NT = N * N * PP;
-    constexpr int IT_T = (NT + kBlock - 1) / kBlock;
-    constexpr int NH = 2 * N * P;
-    constexpr int IT_H = (NH + kBlock - 1) / kBlock;
-    constexpr int NX = N * P;
-    constexpr int IT_X = (NX + kBlock - 1) / kBlock;
-    constexpr int NO = N * N * P;
-    constexpr int IT_O = (NO + kBlock - 1) / kBlock;
-    C rt[IT_T], ru[IT_H], rr[IT_X];
-
-    // Per-thread byte offsets of every load/store slot inside one x element
-    // (pencil-invariant; kOob marks an empty slot).  Tile and u-halo slots
-    // whose node lies in a u/v ghost element are read from fgh instead of fin
-    // (bit it of the gh masks); that only happens on pencils at the velocity
-    // boundary.
-    constexpr int ES = (int)sizeof(T);
-    int o_t[IT_T], o_h[IT_H], o_x[IT_X], o_o[IT_O];
-    unsigned gh_t = 0u, gh_h = 0u;
-#pragma unroll
-    for (int it = 0; it < IT_T; ++it) {
-        const int idx = tid + it * kBlock;
-        o_t[it] = kOob;
-        if (idx < NT) {
-            const int r = idx / PP;
-            const int q = idx - r * PP;
-            if (q <= pvalid + 1) {
-                const int ix = r / N;
-                const int iu = r - ix * N;
-                const int gv = gv0 - 1 + q;
-                const int ev = gv / N;
-                o_t[it] = (ix * sx + base_u + iu * su + gv) * ES;
-                if (ev == 0 || ev == s.nv + 1) gh_t |= 1u << it;
-            }
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < IT_H; ++it) {
-        const int idx = tid + it * kBlock;
-        o_h[it] = kOob;
-        if (idx < NH) {
-            const int side = idx / (N * P);
-            const int rem = idx - side * N * P;
-            const int ix = rem / P;
-            const int q = rem - ix * P;
-            if (q < pvalid) {
-                const int eun = side ? eu + 1 : eu - 1;
-                const int iun = side ? 0 : N - 1;
-                o_h[it] = (ix * sx + eun * sue + iun * su + gv0 + q) * ES;
-                if (eun == 0 || eun == s.nu + 1) gh_h |= 1u << it;
-            }
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < IT_X; ++it) {
-        const int idx = tid + it * kBlock;
-        o_x[it] = kOob;
-        if (idx < NX) {
-            const int iu = idx / P;
-            const int q = idx - iu * P;
-            if (q < pvalid) o_x[it] = (base_u + iu * su + gv0 + q) * ES;
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < IT_O; ++it) {
-        const int idx = tid + it * kBlock;
-        o_o[it] = kOob;
-        if (idx < NO) {
-            const int r = idx / P;
-            const int q = idx - r * P;
-            if (q < pvalid) {
-                const int ix = r / N;
-                const int iu = r - ix * N;
-                o_o[it] = (ix * sx + base_u + iu * su + gv0 + q) * ES;
-            }
-        }
-    }
-    // element-major layout: own elements as 16-byte vectors, v-halo values
-    constexpr int EP = elem_pitch(N);
-    constexpr int VW = ST::VW;
-    constexpr int NVS = (TV * EP) / VW;
-    constexpr int IT_V = (NVS + kBlock - 1) / kBlock;
-    constexpr int NVH = 2 * N * N;
-    constexpr int IT_VH = (NVH + kBlock - 1) / kBlock;
-    const int vrow = s.nv + 2;                       // elements per (ex, eu) row
-    int o_v[L == kElemLayout ? IT_V : 1], o_vh[L == kElemLayout ? IT_VH : 1];
-    unsigned gh_vh = 0u;
-    vl_u32x4 rv[L == kElemLayout ? IT_V : 1];
-    C rvh[L == kElemLayout ? IT_VH : 1];
-    if constexpr (L == kElemLayout) {
-        const int elem0 = (eu * vrow + v0) * EP;
-        const int nvs_valid = nvalid * EP / VW;
-#pragma unroll
-        for (int it = 0; it < IT_V; ++it) {
-            const int slot = tid + it * kBlock;
-            o_v[it] = slot < nvs_valid ? (elem0 + slot * VW) * ES : kOob;
-        }
-#pragma unroll
-        for (int it = 0; it < IT_VH; ++it) {
-            const int idx = tid + it * kBlock;
-            o_vh[it] = kOob;
-            if (idx < NVH) {
-                const int side = idx / (N * N);
-                const int r = idx - side * N * N;
-                const int ev = side ? v0 + nvalid : v0 - 1;
-                o_vh[it] = ((eu * vrow + ev) * EP + r * N + (side ? 0 : N - 1)) * ES;
-                if (ev == 0 || ev == s.nv + 1) gh_vh |= 1u << it;
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < IT_H; ++it) {
-            const int idx = tid + it * kBlock;
-            o_h[it] = kOob;
-            gh_h &= ~(1u << it);
-            if (idx < NH) {
-                const int side = idx / (N * P);
-                const int rem = idx - side * N * P;
-                const int ix = rem / P;
-                const int q = rem - ix * P;
-                if (q < pvalid) {
-                    const int el = q / N;
-                    const int iv = q - el * N;
-                    const int eun = side ? eu + 1 : eu - 1;
-                    const int iun = side ? 0 : N - 1;
-                    o_h[it] = ((eun * vrow + v0 + el) * EP + (ix * N + iun) * N + iv) * ES;
-                    if (eun == 0 || eun == s.nu + 1) gh_h |= 1u << it;
-                }
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < IT_X; ++it) {
-            const int idx = tid + it * kBlock;
-            o_x[it] = kOob;
-            if (idx < NX) {
-                const int iu = idx / P;
-                const int q = idx - iu * P;
-                if (q < pvalid) {
-                    const int el = q / N;
-                    const int iv = q - el * N;
-                    o_x[it] = ((eu * vrow + v0 + el) * EP + iu * N + iv) * ES;
-                }
-            }
-        }
-    }
-    const int xplane_bytes = (L == kElemLayout ? N * N : sx) * ES;   // ix -> ix+1
-    const int64_t xbytes = s.sxe * ES;              // one x element of the state
-    auto rs = [&](const T *base, int e) { return make_rsrc(base + (int64_t)e * s.sxe, xbytes); };
-
-    // tile rows of element e (+ v halo nodes)
-    auto load_tile = [&](int e) {
-        const __amdgpu_buffer_rsrc_t rin = rs(fin, e);
-        if constexpr (L == kElemLayout) {
-#pragma unroll
-            for (int it = 0; it < IT_V; ++it) rv[it] = buf_ld16(rin, o_v[it]);
-#pragma unroll
-            for (int it = 0; it < IT_VH; ++it) rvh[it] = ST::bld(rin, o_vh[it]);
-            if (gh_vh) {
-                const __amdgpu_buffer_rsrc_t rgh = rs(fgh, e);
-#pragma unroll
-                for (int it = 0; it < IT_VH; ++it)
-                    if (gh_vh & (1u << it)) rvh[it] = ST::bld(rgh, o_vh[it]);
-            }
-            return;
-        }
-#pragma unroll
-        for (int it = 0; it < IT_T; ++it) rt[it] = ST::bld(rin, o_t[it]);
-        if (gh_t) {
-            const __amdgpu_buffer_rsrc_t rgh = rs(fgh, e);
-#pragma unroll
-            for (int it = 0; it < IT_T; ++it)
-                if (gh_t & (1u << it)) rt[it] = ST::bld(rgh, o_t[it]);
-        }
-    };
-    // u-face halo of element e: plane iu=N-1 of eu-1 and iu=0 of eu+1
-    auto load_uhalo = [&](int e) {
-        const __amdgpu_buffer_rsrc_t rin = rs(fin, e);
-#pragma unroll
-        for (int it = 0; it < IT_H; ++it) ru[it] = ST::bld(rin, o_h[it]);
-        if (gh_h) {
-            const __amdgpu_buffer_rsrc_t rgh = rs(fgh, e);
-#pragma unroll
-            for (int it = 0; it < IT_H; ++it)
-                if (gh_h & (1u << it)) ru[it] = ST::bld(rgh, o_h[it]);
-        }
-    };
-    // x-plane ixn of element e (interior v nodes of the tile)
-    auto load_xplane = [&](int e, int ixn) {
-        const __amdgpu_buffer_rsrc_t rin = rs(fin, e);
-        const int add = ixn * xplane_bytes;
-#pragma unroll
-        for (int it = 0; it < IT_X; ++it) rr[it] = ST::bld(rin, o_x[it] == kOob ? kOob : o_x[it] + add);
-    };
-    auto store_tile = [&]() {
-        if constexpr (L == kElemLayout) {
-            // scatter the contiguous element runs into the (ix,iu) x q rows
-#pragma unroll
-            for (int it = 0; it < IT_V; ++it) {
-                const int slot = tid + it * kBlock;
-                if (slot < NVS) {
-#pragma unroll
-                    for (int k = 0; k < VW; ++k) {
-                        const int g = slot * VW + k;
-                        const int el = g / EP;
-                        const int rem = g - el * EP;
-                        // elements past nvalid are absent: their slot el*N + 1
-                        // would overwrite the right v-halo node at pvalid + 1
-                        if (rem < N * N * N && el < nvalid) {
-                            const int r = rem / N;            // ix*N + iu
-                            const int iv = rem - r * N;
-                            ft[r * PP + el * N + iv + 1] = ST::unpack(rv[it], k);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int it = 0; it < IT_VH; ++it) {
-                const int idx = tid + it * kBlock;
-                if (idx < NVH) {
-                    const int side = idx / (N * N);
-                    const int r = idx - side * N * N;
-                    ft[r * PP + (side ? pvalid + 1 : 0)] = rvh[it];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int it = 0; it < IT_T; ++it) {
-                const int idx = tid + it * kBlock;
-                if (idx < NT) ft[idx] = rt[it];
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < IT_H; ++it) {
-            const int idx = tid + it * kBlock;
-            if (idx < NH) uh[idx] = ru[it];
-        }
-    };
-    auto store_plane = [&](C *dst) {
-#pragma unroll
-        for (int it = 0; it < IT_X; ++it) {
-            const int idx = tid + it * kBlock;
-            if (idx < NX) dst[idx] = rr[it];
-        }
-    };
-
-    // ---- prologue: first tile, its halos, the background slab ------------
-    load_tile(pc.ex_begin);
-    load_uhalo(pc.ex_begin);
-    load_xplane(xmap(pc.ex_begin - 1), N - 1);
-    if (BG) {
-        const int64_t tbl = (int64_t)(s.nu + 2) * N * (s.nv + 2) * N;
-        for (int idx = tid; idx < 3 * N * P; idx += kBlock) {
-            const int t = idx / (N * P);
-            const int rem = idx - t * N * P;
-            const int iu = rem / P;
-            const int q = rem - iu * P;
-            double v = 0.0;
-            if (q < pvalid) v = s.bg[t * tbl + ((int64_t)eu * N + iu) * (s.nv + 2) * N + gv0 + q];
-            bgs[idx] = v;
-        }
-    }
-    store_tile();
-    store_plane(xl);
-    load_xplane(xmap(pc.ex_begin + 1), 0);
-    store_plane(xr);
-    __syncthreads();
-
-    const int tiles_per_ex = s.nu * s.nchunk;
-    for (int ex = pc.ex_begin; ex <= pc.ex_end; ++ex) {
-        const bool has_next = ex < pc.ex_end;
-        const __amdgpu_buffer_rsrc_t rfn = rs(fn, ex);
-        const __amdgpu_buffer_rsrc_t rout = rs(fout, ex);
-        // prefetch the next tile, its u halo and the right x halo of the next step
-        double e_next = 0.0;
-        if (has_next) {
-            load_tile(ex + 1);
-            load_uhalo(ex + 1);
-            load_xplane(xmap(ex + 2), 0);
-            if (lx_on) e_next = E[(ex + 1) * N + lx_iu];
-        }
-        C rn[L == kElemLayout ? 1 : IT_O];
-        vl_u32x4 rnv[L == kElemLayout ? IT_V : 1];
-        if (MODE == kModeStage) {
-            if constexpr (L == kElemLayout) {
-#pragma unroll
-                for (int it = 0; it < IT_V; ++it) rnv[it] = buf_ld16(rfn, o_v[it]);
-            } else {
-#pragma unroll
-                for (int it = 0; it < IT_O; ++it) rn[it] = ST::bld(rfn, o_o[it]);
-            }
-        }
-
-        // ---- x-lines: (iu, q), line over ix ------------------------------
-        if (lx_on) {
-            const int iu = lx_iu;
-            const int q = lx_q;
-            {
-                const double ad = ax;
-                C f[N], o[N];
-#pragma unroll
-                for (int j = 0; j < N; ++j) f[j] = ft[(j * N + iu) * PP + q + 1];
-                line_op<N, C>(f, xl[iu * P + q], xr[iu * P + q], (C)ad, ad >= 0.0, tabp, o);
-#pragma unroll
-                for (int i = 0; i < N; ++i) acc_l[(i * N + iu) * P + q] = o[i] * (C)s.jx;
-            }
-        }
-        __syncthreads();
-        // ---- u-lines: (ix, q), line over iu ------------------------------
-        if (lx_on) {
-            const int ix = lx_iu;
-            const int q = lx_q;
-            {
-                const double ad = s.acc * (e_cur + bv);
-                const bool pos = ad >= 0.0;
-                C f[N], o[N];
-#pragma unroll
-                for (int j = 0; j < N; ++j) f[j] = ft[(ix * N + j) * PP + q + 1];
-                line_op<N, C>(f, uh[ix * P + q], uh[(N + ix) * P + q], (C)ad, pos, tabp, o);
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    C add = o[i] * (C)s.ju;
-                    if (BG) add += (C)(s.scale * (ad * bgs[(pos ? 0 : N * P) + i * P + q]));
-                    acc_l[(ix * N + i) * P + q] += add;
-                }
-            }
-        }
-        __syncthreads();
-        // ---- v-lines: (ix, iu, ev_local), line over iv -------------------
-        if (lv_on) {
-            const int r = lv_r;
-            const int el = lv_el;
-            {
-                const int iu = r - (r / N) * N;
-                const double ad = av;
-                const C *row = &ft[r * PP + el * N];
-                C f[N], o[N];
-#pragma unroll
-                for (int j = 0; j < N; ++j) f[j] = row[j + 1];
-                line_op<N, C>(f, row[0], row[N + 1], (C)ad, ad >= 0.0, tabp, o);
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    C add = o[i] * (C)s.jv;
-                    if (BG) add += (C)(s.scale * bgs[2 * N * P + iu * P + el * N + i]);
-                    acc_l[r * P + el * N + i] += add;
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---- epilogue: combine and write rows (contiguous along v) -------
-        // fn and fout may be the same array: each element is read and then
-        // written by the same thread.
-        if constexpr (L == kElemLayout) {
-#pragma unroll
-            for (int it = 0; it < IT_V; ++it) {
-                const int slot = tid + it * kBlock;
-                if (slot < NVS) {
-                    vl_u32x4 w = {0u, 0u, 0u, 0u};
-#pragma unroll
-                    for (int k = 0; k < VW; ++k) {
-                        const int g = slot * VW + k;
-                        const int el = g / EP;
-                        const int rem = g - el * EP;
-                        if (rem < N * N * N) {
-                            const int r = rem / N;
-                            const int iv = rem - r * N;
-                            const int q = el * N + iv;
-                            const C Lr = acc_l[r * P + q];
-                            C val;
-                            if (MODE == kModeRhs) {
-                                val = Lr;
-                            } else if (MODE == kModeStage1) {
-                                val = ft[r * PP + q + 1] + (C)s.c2dt * Lr;
-                            } else {
-                                val = ((C)s.c0 * ST::unpack(rnv[it], k) + (C)s.c1 * ft[r * PP + q + 1]) +
-                                      (C)s.c2dt * Lr;
-                            }
-                            ST::pack(w, k, val);
-                            acc_l[r * P + q] = ST::round(val);
-                        }
-                    }
-                    buf_st16(rout, o_v[it], w);     // slots past the valid elements are dropped
-                }
-            }
-        } else {
-#pragma unroll
-        for (int it = 0; it < IT_O; ++it) {
-            const int idx = tid + it * kBlock;
-            if (idx < NO) {
-                const int r = idx / P;
-                const int q = idx - r * P;
-                const C Lr = acc_l[idx];
-                C val;
-                if (MODE == kModeRhs) {
-                    val = Lr;
-                } else if (MODE == kModeStage1) {
-                    val = ft[r * PP + q + 1] + (C)s.c2dt * Lr;
-                } else {
-                    val = ((C)s.c0 * rn[it] + (C)s.c1 * ft[r * PP + q + 1]) + (C)s.c2dt * Lr;
-                }
-                ST::bst(rout, o_o[it], val);      // empty slots (q >= pvalid) are dropped
-                acc_l[idx] = ST::round(val);     // the moment sees the value as stored
-            }
-        }
-        }
-        __syncthreads();
-
-        // ---- moment partial of the written values; left halo of next tile
-        if (MODE != kModeRhs && s.write_partials) {
-            // plain FP64 inside the tile (a few hundred terms); the sums over
-            // tiles are double-double in the reduction kernel
-            if (tid < N * P) {
-                const int ix = tid / P;
-                const int q = tid - ix * P;
-                double acc = 0.0;
-                if (q < pvalid) {
-                    const int iv = q - (q / N) * N;
-                    ConstOpTables &tab = fresh(tabp);
-#pragma unroll
-                    for (int iu = 0; iu < N; ++iu)
-                        acc += (double)acc_l[(ix * N + iu) * P + q] * tab.wq[iu * N + iv];
-                }
-                red[tid] = acc;
-            }
-            __syncthreads();
-            // fixed two-level tree over the P columns of each x node
-            constexpr int G = 4;
-            constexpr int NG = (P + G - 1) / G;
-            const bool g_on = tid < N * NG;
-            const int gix = g_on ? tid / NG : 0;
-            const int gg = g_on ? tid - gix * NG : 0;
-            double gacc = 0.0;
-            if (g_on) {
-#pragma unroll
-                for (int k = 0; k < G; ++k) {
-                    const int q = gg * G + k;
-                    if (q < P) gacc += red[gix * P + q];
-                }
-            }
-            __syncthreads();
-            if (g_on) red[gix * P + gg] = gacc;
-            __syncthreads();
-            if (tid < N) {
-                double acc = 0.0;
-                for (int g = 0; g < NG; ++g) acc += red[tid * P + g];
-                const int64_t slot =
-                    ((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * N + tid;
-                s.partials[slot] = make_double2(BG ? acc * s.inv_scale : acc, 0.0);
-            }
-        }
-        e_cur = e_next;
-        if (has_next) {
-            if (tid < N * P) {
-                const int iu = tid / P;
-                const int q = tid - iu * P;
-                xl[tid] = ft[((N - 1) * N + iu) * PP + q + 1];
-            }
-            __syncthreads();
-            store_tile();
-            store_plane(xr);
-            __syncthreads();
-        }
-    }
+    __shared__ typename Store<S>::C ft[StageLds<N, S>::n_ft];
+    __shared__ typename Store<S>::C xl[StageLds<N, S>::n_x];
+    __shared__ typename Store<S>::C xr[StageLds<N, S>::n_x];
+    __shared__ typename Store<S>::C uh[StageLds<N, S>::n_uh];
+    __shared__ typename Store<S>::C acc_l[StageLds<N, S>::n_acc];
+    __shared__ double bgs[StageLds<N, S>::n_bgs];
+    __shared__ double red_own[StageLds<N, S>::n_red];
+    const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
+    const int stage_tid = threadIdx.x;
+#include "vl_stage_body.inc"
 }
 
 // Moment partials of a state array (no RHS), one workgroup per tile

@@ -17,6 +17,7 @@
 #include "vl_stage.h"
 #include "vl_stage_mx.h"
 #include "vl_stage_rs.h"
+#include "vl_ensemble.h"
 
 #include <vector>
 
@@ -940,6 +941,372 @@ int vl_set_background(vl_geom_t h, const double *f0_host) {
     h->f0_host.assign(f0_host, f0_host + cnt);
     if (int r = upload_background(h)) return r;
     return VL_OK;
+}
+
+}  // extern "C"
+
+// ---- ensembles of runs ----------------------------------------------------
+
+struct vl_ensemble {
+    int count, storage;
+    std::vector<vl_geom *> geoms;
+    std::vector<EnsRun> runs;      // host copy of the device table
+    std::vector<char> bound;
+    EnsRun *d_runs;
+    // schedule of the stage kernel
+    int work;                      // target DOF per work item (sets each run's x-segment)
+    int grid;                      // workgroups of the persistent launch (0: one per item)
+    int order;                     // 0: longest items first; 1: run by run
+    std::vector<int2> items;
+    int2 *d_items;
+    size_t items_cap;
+    // field kernels: workgroup maps
+    EnsBlock *d_map_swap, *d_map_red, *d_map_fwd, *d_map_inv;
+    int nb_swap, nb_red, nb_fwd, nb_inv;
+    size_t fwd_lds, inv_lds;
+    bool dirty;                    // host tables changed since the last upload
+};
+
+namespace {
+
+constexpr int kEnsSwapCap = 2048;     // swap workgroups per run (grid-stride beyond)
+
+int ens_check(const vl_ensemble *e) {
+    if (!e) return fail(VL_E_HANDLE, "null ensemble handle");
+    return 0;
+}
+
+// x elements per pencil of run r so that a pencil carries about `work` DOF
+int ens_xseg(const vl_geom *h, int work) {
+    const int64_t per_x = (int64_t)h->TV * h->n * h->n * h->n;
+    int64_t xs = (work + per_x / 2) / per_x;
+    xs = std::max<int64_t>(1, std::min<int64_t>(xs, h->nx));
+    return (int)xs;
+}
+
+// Work list and per-run x-segments for the current schedule.
+void ens_build_items(vl_ensemble *e) {
+    struct Item {
+        int run, pencil;
+        int64_t cost;
+    };
+    std::vector<Item> all;
+    for (int r = 0; r < e->count; ++r) {
+        const vl_geom *h = e->geoms[r];
+        const int xs = ens_xseg(h, e->work);
+        e->runs[r].s.xseg = xs;
+        const int base = h->nu * h->nchunk;
+        const int nseg = (h->nx + xs - 1) / xs;
+        const int64_t per_x = (int64_t)h->TV * h->n * h->n * h->n;
+        for (int p = 0; p < base * nseg; ++p) {
+            const int seg = p / base;
+            const int len = std::min(xs, h->nx - seg * xs);
+            all.push_back({r, p, len * per_x});
+        }
+    }
+    if (e->order == 0)
+        std::stable_sort(all.begin(), all.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+    e->items.resize(all.size());
+    for (size_t i = 0; i < all.size(); ++i) e->items[i] = make_int2(all[i].run, all[i].pencil);
+    e->dirty = true;
+}
+
+int ens_upload(vl_ensemble *e) {
+    if (!e->dirty) return VL_OK;
+    VL_HIP(hipMemcpy(e->d_runs, e->runs.data(), e->runs.size() * sizeof(EnsRun), hipMemcpyHostToDevice));
+    if (e->items.size() > e->items_cap) {
+        if (e->d_items) VL_HIP(hipFree(e->d_items));
+        e->d_items = nullptr;
+        VL_HIP(hipMalloc((void **)&e->d_items, e->items.size() * sizeof(int2)));
+        e->items_cap = e->items.size();
+    }
+    VL_HIP(hipMemcpy(e->d_items, e->items.data(), e->items.size() * sizeof(int2), hipMemcpyHostToDevice));
+    e->dirty = false;
+    return VL_OK;
+}
+
+int ens_ready(vl_ensemble *e, const char *what) {
+    for (int r = 0; r < e->count; ++r)
+        if (!e->bound[r]) return fail(VL_E_ARG, "%s: run %d has no buffers (vl_ensemble_bind)", what, r);
+    return ens_upload(e);
+}
+
+template <int S>
+hipError_t ens_stage_launch(const vl_ensemble *e, int stage, hipStream_t st) {
+    const int n_items = (int)e->items.size();
+    const int G = e->grid > 0 ? std::min(e->grid, n_items) : n_items;
+    if (stage == 1)
+        hipLaunchKernelGGL((k_ens_stage<kModeStage1, S>), dim3(G), dim3(kBlock), 0, st, e->d_runs, e->d_items,
+                           n_items, stage);
+    else
+        hipLaunchKernelGGL((k_ens_stage<kModeStage, S>), dim3(G), dim3(kBlock), 0, st, e->d_runs, e->d_items,
+                           n_items, stage);
+    return hipGetLastError();
+}
+
+hipError_t ens_stage(const vl_ensemble *e, int stage, hipStream_t st) {
+    return e->storage == VL_F64 ? ens_stage_launch<VL_F64>(e, stage, st) : ens_stage_launch<VL_F32>(e, stage, st);
+}
+
+// moment partials -> rho -> E of every run (energy: one value per run, or null)
+hipError_t ens_field(const vl_ensemble *e, double *energy, hipStream_t st) {
+    hipLaunchKernelGGL(k_ens_reduce, dim3(e->nb_red), dim3(kBlock), 0, st, e->d_runs, e->d_map_red);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_ens_forward, dim3(e->nb_fwd), dim3(kBlock), e->fwd_lds, st, e->d_runs, e->d_map_fwd);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_ens_inverse, dim3(e->nb_inv), dim3(kBlock), e->inv_lds, st, e->d_runs, e->d_map_inv,
+                       energy);
+    return hipGetLastError();
+}
+
+hipError_t ens_swap(const vl_ensemble *e, hipStream_t st) {
+    if (e->storage == VL_F64)
+        hipLaunchKernelGGL(k_ens_swap<double>, dim3(e->nb_swap), dim3(kBlock), 0, st, e->d_runs, e->d_map_swap);
+    else
+        hipLaunchKernelGGL(k_ens_swap<float>, dim3(e->nb_swap), dim3(kBlock), 0, st, e->d_runs, e->d_map_swap);
+    return hipGetLastError();
+}
+
+hipError_t ens_step(const vl_ensemble *e, double *energy, hipStream_t st) {
+    hipError_t err = ens_swap(e, st);
+    for (int stage = 1; stage <= 3 && err == hipSuccess; ++stage) {
+        err = ens_field(e, stage == 3 ? energy : nullptr, st);
+        if (err == hipSuccess) err = ens_stage(e, stage, st);
+    }
+    return err;
+}
+
+// moment partials of every run's f (the first stage of a loop needs them)
+int ens_moments(vl_ensemble *e, hipStream_t st) {
+    for (int r = 0; r < e->count; ++r) {
+        const vl_geom *h = e->geoms[r];
+        VL_HIP(launch_moment(h, e->storage, kElemLayout, e->runs[r].f, e->runs[r].s, st));
+    }
+    return VL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vl_ensemble_create(const vl_geom_t *geoms, int count, vl_ensemble_t *out) {
+    if (!geoms || !out) return fail(VL_E_NULL, "vl_ensemble_create: null pointer");
+    *out = nullptr;
+    if (count < 1) return fail(VL_E_ARG, "vl_ensemble_create: count must be >= 1");
+    const int storage = geoms[0] ? geoms[0]->storage : -1;
+    if (storage != VL_F64 && storage != VL_F32)
+        return fail(VL_E_ARG, "vl_ensemble_create: FP64 or FP32 storage only");
+    for (int r = 0; r < count; ++r) {
+        const vl_geom *h = geoms[r];
+        if (!h) return fail(VL_E_HANDLE, "vl_ensemble_create: null geometry %d", r);
+        if (h->storage != storage) return fail(VL_E_ARG, "vl_ensemble_create: runs must share one storage");
+        if (h->n < kEnsMinN || h->n > kEnsMaxN)
+            return fail(VL_E_ARG, "vl_ensemble_create: run %d has n=%d outside [%d,%d]", r, h->n, kEnsMinN,
+                        kEnsMaxN);
+        if (storage != VL_F64 && !h->d_bg)
+            return fail(VL_E_ARG, "vl_ensemble_create: run %d needs vl_set_background first", r);
+    }
+    vl_ensemble *e = new (std::nothrow) vl_ensemble();
+    if (!e) return fail(VL_E_ARG, "out of host memory");
+    e->count = count;
+    e->storage = storage;
+    e->geoms.assign(geoms, geoms + count);
+    e->runs.assign(count, EnsRun{});
+    e->bound.assign(count, 0);
+    e->work = 9000;
+    e->grid = 0;
+    e->order = 0;
+    std::vector<EnsBlock> mswap, mred, mfwd, minv;
+    size_t fwd_lds = 8, inv_lds = 16;
+    for (int r = 0; r < count; ++r) {
+        const vl_geom *h = geoms[r];
+        EnsRun &x = e->runs[r];
+        x.s = make_stage_args(h, kElemLayout);
+        x.s.periodic_x = 1;
+        x.s.write_partials = 1;
+        x.tab = h->d_tab;
+        x.unodes = h->d_unodes;
+        x.vnodes = h->d_vnodes;
+        x.wts = h->d_wts;
+        x.kwave = h->d_kwave;
+        x.T = h->d_T;
+        x.phases = h->d_phases;
+        x.fa = make_field_args(h, 1);
+        x.plane = h->sxe_elem;
+        x.n = h->n;
+        x.tpe = tiles_per_ex_of(h, kElemLayout);
+        x.inv_blocks = h->inv_blocks;
+        x.swap_blocks = (int)std::min<int64_t>((2 * x.plane + kBlock - 1) / kBlock, kEnsSwapCap);
+        for (int b = 0; b < x.swap_blocks; ++b) mswap.push_back({r, b});
+        for (int b = 0; b < h->nx; ++b) mred.push_back({r, b});
+        for (int b = 0; b < (h->M + kDftModes - 1) / kDftModes; ++b) mfwd.push_back({r, b});
+        for (int b = 0; b < h->inv_blocks; ++b) minv.push_back({r, b});
+        fwd_lds = std::max(fwd_lds, (size_t)h->nx * h->n * sizeof(double));
+        inv_lds = std::max(inv_lds, (size_t)2 * h->M * sizeof(double2));
+    }
+    e->fwd_lds = fwd_lds;
+    e->inv_lds = inv_lds;
+    e->nb_swap = (int)mswap.size();
+    e->nb_red = (int)mred.size();
+    e->nb_fwd = (int)mfwd.size();
+    e->nb_inv = (int)minv.size();
+    hipError_t err = hipMalloc((void **)&e->d_runs, count * sizeof(EnsRun));
+    if (err == hipSuccess) err = alloc_copy(&e->d_map_swap, mswap.data(), mswap.size());
+    if (err == hipSuccess) err = alloc_copy(&e->d_map_red, mred.data(), mred.size());
+    if (err == hipSuccess) err = alloc_copy(&e->d_map_fwd, mfwd.data(), mfwd.size());
+    if (err == hipSuccess) err = alloc_copy(&e->d_map_inv, minv.data(), minv.size());
+    if (err != hipSuccess) {
+        vl_ensemble_destroy(e);
+        return hip_fail(err, "vl_ensemble_create");
+    }
+    ens_build_items(e);
+    *out = e;
+    return VL_OK;
+}
+
+int vl_ensemble_destroy(vl_ensemble_t e) {
+    if (!e) return VL_OK;
+    void *bufs[] = {e->d_runs, e->d_items, e->d_map_swap, e->d_map_red, e->d_map_fwd, e->d_map_inv};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete e;
+    return VL_OK;
+}
+
+int vl_ensemble_bind(vl_ensemble_t e, int run, void *f, void *f1, void *f2, double *E, void *ws, double dt) {
+    if (int r = ens_check(e)) return r;
+    if (run < 0 || run >= e->count) return fail(VL_E_ARG, "vl_ensemble_bind: run %d outside [0,%d)", run, e->count);
+    if (!f || !f1 || !f2 || !E || !ws) return fail(VL_E_NULL, "vl_ensemble_bind: null pointer");
+    if (!aligned(f, 16) || !aligned(f1, 16) || !aligned(f2, 16) || !aligned(E, 8) || !aligned(ws, 256))
+        return fail(VL_E_ALIGN, "vl_ensemble_bind: misaligned pointer");
+    if (!std::isfinite(dt)) return fail(VL_E_NOTFINITE, "vl_ensemble_bind: dt not finite");
+    const vl_geom *h = e->geoms[run];
+    EnsRun &x = e->runs[run];
+    x.f = f;
+    x.f1 = f1;
+    x.f2 = f2;
+    x.E = E;
+    x.s.partials = ws_partials(h, ws);
+    x.s.seed = ws_seed(h, ws);
+    x.rho = ws_rho(h, ws);
+    x.coef = ws_coef(h, ws);
+    x.enpart = ws_enpart(h, ws);
+    x.mxpart = ws_mxpart(h, ws);
+    x.ticket = ws_ticket(h, ws);
+    x.seed = ws_seed(h, ws);
+    // the Shu-Osher coefficients as vl_rk3_step forms them (main/timestep.py:37-38)
+    const double cf[3][3] = {{0.0, 1.0, dt}, {3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt},
+                             {1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt}};
+    std::memcpy(x.cf, cf, sizeof(cf));
+    e->bound[run] = 1;
+    e->dirty = true;
+    return VL_OK;
+}
+
+int vl_ensemble_set_schedule(vl_ensemble_t e, int work_per_item, int grid, int order) {
+    if (int r = ens_check(e)) return r;
+    if (work_per_item < 1) return fail(VL_E_ARG, "vl_ensemble_set_schedule: work_per_item must be >= 1");
+    if (grid < 0) return fail(VL_E_ARG, "vl_ensemble_set_schedule: grid must be >= 0");
+    if (order != 0 && order != 1) return fail(VL_E_ARG, "vl_ensemble_set_schedule: order must be 0 or 1");
+    e->work = work_per_item;
+    e->grid = grid;
+    e->order = order;
+    ens_build_items(e);
+    return VL_OK;
+}
+
+int vl_ensemble_get_schedule(vl_ensemble_t e, int *work_per_item, int *grid, int *order, int *items) {
+    if (int r = ens_check(e)) return r;
+    if (work_per_item) *work_per_item = e->work;
+    if (grid) *grid = e->grid;
+    if (order) *order = e->order;
+    if (items) *items = (int)e->items.size();
+    return VL_OK;
+}
+
+int vl_ensemble_moments(vl_ensemble_t e, void *stream) {
+    if (int r = ens_check(e)) return r;
+    if (int r = ens_ready(e, "vl_ensemble_moments")) return r;
+    return ens_moments(e, (hipStream_t)stream);
+}
+
+int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream) {
+    if (int r = ens_check(e)) return r;
+    if (energy_out && !aligned(energy_out, 8)) return fail(VL_E_ALIGN, "vl_ensemble_rk3_step: misaligned energy");
+    if (int r = ens_ready(e, "vl_ensemble_rk3_step")) return r;
+    VL_HIP(ens_step(e, energy_out, (hipStream_t)stream));
+    return VL_OK;
+}
+
+int vl_ensemble_autotune(vl_ensemble_t e, int reps, float *table, int max_rows, int *rows, void *stream) {
+    if (int r = ens_check(e)) return r;
+    if (reps < 1) return fail(VL_E_ARG, "vl_ensemble_autotune: reps must be >= 1");
+    if (int r = ens_ready(e, "vl_ensemble_autotune")) return r;
+    hipStream_t st = (hipStream_t)stream;
+    int cus = 256;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+    }
+    const int works[] = {4500, 9000, 18000, 36000, 72000};
+    const int grids[] = {0, 3 * cus, 6 * cus, 12 * cus};
+    hipEvent_t t0, t1;
+    VL_HIP(hipEventCreate(&t0));
+    VL_HIP(hipEventCreate(&t1));
+    float best = 1e30f;
+    int bw = e->work, bg = e->grid, bo = e->order, nrow = 0;
+    int rc = VL_OK;
+    for (int o = 0; o < 2 && rc == VL_OK; ++o)
+        for (int w : works)
+            for (int g : grids) {
+                e->work = w;
+                e->grid = g;
+                e->order = o;
+                ens_build_items(e);
+                if ((rc = ens_upload(e)) != VL_OK) break;
+                // stage 2 reads f1 and f and writes f2 (and the moment
+                // partials); f is not touched
+                hipError_t err = ens_stage(e, 2, st);
+                if (err == hipSuccess) err = hipEventRecord(t0, st);
+                for (int k = 0; k < reps && err == hipSuccess; ++k) err = ens_stage(e, 2, st);
+                if (err == hipSuccess) err = hipEventRecord(t1, st);
+                if (err == hipSuccess) err = hipEventSynchronize(t1);
+                float ms = 0.f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+                if (err != hipSuccess) {
+                    rc = hip_fail(err, "vl_ensemble_autotune");
+                    break;
+                }
+                ms /= reps;
+                if (table && nrow < max_rows) {
+                    table[4 * nrow + 0] = (float)w;
+                    table[4 * nrow + 1] = (float)g;
+                    table[4 * nrow + 2] = (float)o;
+                    table[4 * nrow + 3] = ms;
+                }
+                ++nrow;
+                if (ms < best) {
+                    best = ms;
+                    bw = w;
+                    bg = g;
+                    bo = o;
+                }
+            }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rows) *rows = nrow;
+    e->work = bw;
+    e->grid = bg;
+    e->order = bo;
+    ens_build_items(e);
+    if (rc != VL_OK) return rc;
+    if (int r = ens_upload(e)) return r;
+    // the timed stages overwrote the moment partials: those of f again
+    return ens_moments(e, st);
 }
 
 }  // extern "C"
