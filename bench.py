@@ -1,6 +1,6 @@
 """RK3 DOF-updates/s of the GPU SSP-RK3 path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload ring64] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload ring64] [--no-cpu-baseline] [--ensemble]
 
 A step is one full Shu-Osher SSP-RK3 step (3 fused stage kernels + moment
 reduction + spectral Poisson solve per stage) over the 116.64M-DOF workload
@@ -64,6 +64,9 @@ def main():
                     help="x elements each stage-kernel workgroup walks (0: the library default)")
     ap.add_argument("--no-mfma", action="store_true",
                     help="run the BF16/FP16 stages on the vector ALUs instead of the matrix cores")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="also time the 48-run mixed-shape FP32 ensemble (one persistent launch per "
+                         "RK stage for all runs) and add it as `ensemble`")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -132,6 +135,12 @@ def main():
     }
     if "bf16" in forms and "f64" in forms:
         out["bf16_over_f64"] = forms["bf16"]["dof_updates_per_s"] / forms["f64"]["dof_updates_per_s"]
+    if args.ensemble:
+        from vlasov1d2v_amd.ensemble import run_workload
+        torch.cuda.empty_cache()
+        ens = run_workload(runs=48, seed=0, steps=200, graph_steps=20, storage="f32")
+        out["ensemble"] = {k: ens[k] for k in ("runs", "dof", "storage", "steps", "dof_updates_per_s",
+                                               "ms_per_step", "schedule", "launches_per_step")}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

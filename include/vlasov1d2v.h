@@ -202,6 +202,25 @@ enum { VL_STEP_RHO_VALID = 1, VL_STEP_ELEMENT_MAJOR = 2, VL_STEP_ROW_SPLIT = 4 }
 int vl_rk3_step(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi,
                 double dt, double *energy_out, void *workspace, int flags, void *stream);
 
+/* Adaptive time step on the device (main/timestep.py:88-89,184,231-242:
+ * dt = courant / (max|u|/dx + (max|acc E| + |acc B max|v||)/du
+ *                 + |acc |B max|u|||/dv), recomputed from the field of the
+ * last stage solve after every step, and time += the new dt), so that
+ * adaptive runs need no host synchronisation and can be graph-captured.
+ * max|u|, max|v| are taken over the ghosted nodes given to vl_geom_create;
+ * acc and B are the handle's (vl_set_physics).
+ * vl_set_cfl: the Courant number and the cell sizes dx, du, dv.
+ * vl_clock_init: the workspace clock gets time `time` and the dt of the
+ *   field E ((nx+2)*n, e.g. the set-up solve's) -- the loop-entry adapt.
+ * vl_rk3_step_adaptive: vl_rk3_step with the clock's dt; afterwards the
+ *   clock holds the adapted dt and the advanced time, and clock_out (device,
+ *   may be NULL) receives (time, dt) -- the values main/timestep.py:96-98
+ *   records after the step. */
+int vl_set_cfl(vl_geom_t h, double courant, double dx, double du, double dv);
+int vl_clock_init(vl_geom_t h, const double *E, double time, void *workspace, void *stream);
+int vl_rk3_step_adaptive(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi, double *energy_out,
+                         double *clock_out, void *workspace, int flags, void *stream);
+
 /* Optional timing hook (no counterpart in the reference, whose only timing
  * is a printed wall clock, main/timestep.py:117,139): when `events` is
  * non-NULL, every later vl_rk3_step on this handle records events[2k] just
@@ -256,7 +275,7 @@ int vl_ensemble_moments(vl_ensemble_t e, void *stream);
 int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream);
 
 /* Autotuner: times the stage-2 launch (reps times each) over x-segment work
- * targets {4500 .. 72000 DOF}, persistent grids {one per item, 3, 6, 12 per
+ * targets {9000 .. 288000 DOF, factors of 2}, persistent grids {one per item, 3, 6, 12 per
  * CU} and both item orders; keeps the fastest schedule, then restores the
  * moment partials of f.  table (may be NULL): up to max_rows rows of
  * (work, grid, order, ms); *rows = rows measured.  Synchronises the stream. */

@@ -14,8 +14,9 @@ the workload definition:
   give inf -> 1.8e308 (main/elliptic.py:124-128).
 * ``math.factorial`` replaces the removed ``np.math.factorial``
   (main/grid.py:253).
-* The time step is fixed (the CFL rule of main/timestep.py:184,231-242 is
-  available as ``get_max_speeds``/``adapt_dt`` but not applied by default).
+* The time step is fixed unless ``Stepper.main_loop`` is given a Courant
+  number; then it follows the CFL rule of main/timestep.py:88-89,184,231-242
+  (``get_max_speeds``/``adapt_dt``).
 * Lobatto nodes/weights are computed by the Newton iteration of
   Hesthaven & Warburton (``JacobiGL``-style, von Winckel's ``lglnodes``) in
   80-bit long double and rounded to FP64; they equal the literal tables of
@@ -46,6 +47,11 @@ Pinning (tests/test_oracle_*.py):
 * ``Stepper``          -- SSP-RK3 amplification polynomial on a linear
                           problem, Shu-Osher table.
 * ``electric_energy``  -- closed form for E = A cos(kx).
+* ``get_max_speeds`` / ``adapt_dt`` -- the rule's value at the 2.1M-DOF
+                          set-up (0.00944, computed independently from the
+                          grid and field) and the zero-field closed form; the
+                          loop order (adapt after the step, time += new dt)
+                          follows the reference text.
 The composite conventions (frozen u/v ghosts, lagged energy series) follow
 the reference text; no reference-held vectors exist to pin them further.
 """
@@ -383,6 +389,10 @@ def get_max_speeds(grids, elliptic, acc):                   # main/timestep.py:2
         acc * abs(elliptic.magnetic_field * grids.u.arr_max)]))
 
 
+# Courant numbers of the 3-stage row, n = 1..9 (main/timestep.py:24-33)
+COURANT_NUMBERS_RK3 = [1.256, 0.409, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033]
+
+
 def adapt_dt(courant, max_speeds, dx, du, dv):              # main/timestep.py:231-232
     return courant / ((max_speeds[0] / dx) + (max_speeds[1] / du) + (max_speeds[2] / dv))
 
@@ -425,10 +435,22 @@ class Stepper:
             arr_stages[g_i] = c[0] * distribution.arr[ng] + c[1] * arr_stages[g_i1] + c[2] * self.dt * df_dt
         distribution.arr[ng] = arr_stages[-1, 1:-1, :, 1:-1, :, 1:-1, :]
 
-    def main_loop(self, steps, distribution, basis, elliptic, grids, dg_flux):  # main/timestep.py:81-139
+    def main_loop(self, steps, distribution, basis, elliptic, grids, dg_flux, courant=None, acc=None,
+                  final_time=None):  # main/timestep.py:81-139
+        """Fixed dt, or with ``courant`` the reference's adaptive rule: dt from
+        the set-up field before the loop (main/timestep.py:88-89), again after
+        every step from the field of the last stage solve (:184), and the time
+        advanced by that new dt (:96).  ``final_time``: loop while time <
+        final_time instead of ``steps`` steps."""
         self.field_energy = np.append(self.field_energy, elliptic.electric_energy(grids.x))
-        for _ in range(steps):
+        adapt = (lambda: adapt_dt(courant, get_max_speeds(grids, elliptic, acc), grids.x.dx, grids.u.dx,
+                                  grids.v.dx)) if courant is not None else None
+        if adapt:
+            self.dt = adapt()
+        while (self.time < final_time) if final_time is not None else (self.steps_counter < steps):
             self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux)
+            if adapt:
+                self.dt = adapt()
             self.time += self.dt
             self.steps_counter += 1
             self.time_array = np.append(self.time_array, self.time)
@@ -468,9 +490,13 @@ class Case:
         arr = self.dist.arr if arr is None else arr
         return self.flux.semi_discrete_rhs(arr, self.elliptic, self.basis, self.grids)
 
-    def run(self, steps, dt):
+    def run(self, steps, dt, final_time=None):
+        """``steps`` steps of a fixed dt, or with dt=None the adaptive rule
+        (Courant number of the 3-stage row for n nodes)."""
         st = Stepper(dt)
-        st.main_loop(steps, self.dist, self.basis, self.elliptic, self.grids, self.flux)
+        courant = COURANT_NUMBERS_RK3[self.n - 1] if dt is None else None
+        st.main_loop(steps, self.dist, self.basis, self.elliptic, self.grids, self.flux, courant=courant,
+                     acc=self.acc, final_time=final_time)
         return st
 
 

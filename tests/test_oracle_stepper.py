@@ -80,3 +80,47 @@ def test_energy_series_is_lagged():
     st = case.run(2, 0.01)
     assert len(st.field_energy) == 3
     assert st.field_energy[-1] == case.elliptic.electric_energy(case.grids.x)
+
+
+def test_adaptive_dt_known_answer_at_the_reference_size_setup():
+    """The CFL rule (main/timestep.py:231-242) on the 2.1M-DOF workload's grid
+    (n=8, 16^3 elements, k0=0.1, B=0.1, [-8.5,8.5]^2): max|u| = max|v| =
+    8.5 + du over the ghosted nodes (du = 17/16), dx = 2 pi / (0.1 * 16),
+    Courant number 0.040 -> 0.00944 for a vanishing field (the rule's value
+    quoted for this workload); the set-up field, E ~ 1e-3/k0 from the 1e-3
+    density modulation, adds max|acc E| ~ 0.01 to the u speed."""
+    case = oracle.Case(n=8, nx=16, nu=16, nv=16)
+    sp = oracle.get_max_speeds(case.grids, case.elliptic, case.acc)
+    um = 8.5 + 17 / 16
+    assert sp[0] == pytest.approx(um, rel=1e-15)
+    assert sp[1] - 0.1 * um == pytest.approx(1e-3 / 0.1, rel=0.02)
+    dt = oracle.adapt_dt(oracle.COURANT_NUMBERS_RK3[7], sp, case.grids.x.dx, case.grids.u.dx, case.grids.v.dx)
+    # zero field: the closed form with speeds (max|u|, |B| max|v|, |B| max|u|)
+    dx, du = 2 * math.pi / (0.1 * 16), 17 / 16
+    dt0 = 0.040 / (um / dx + 0.1 * um / du + 0.1 * um / du)
+    assert round(dt0, 5) == 0.00944
+    case.elliptic.electric_field = np.zeros_like(case.elliptic.electric_field)
+    sp0 = oracle.get_max_speeds(case.grids, case.elliptic, case.acc)
+    assert oracle.adapt_dt(0.040, sp0, case.grids.x.dx, case.grids.u.dx, case.grids.v.dx) == pytest.approx(dt0, rel=1e-14)
+    assert dt == pytest.approx(0.040 / (0.040 / dt0 + (sp[1] - 0.1 * um) / du), rel=1e-14) and dt < dt0
+
+
+def test_adaptive_loop_advances_time_by_the_new_dt():
+    """main/timestep.py:184 adapts dt at the end of the step and :96 then adds
+    the new dt: time_array[k] = sum of the dts adapted after steps 1..k."""
+    case = oracle.Case(n=3, nx=3, nu=3, nv=3)
+    st = oracle.Stepper(None)
+    dts = []
+    from oracle import vlasov_oracle as vo
+    orig = vo.adapt_dt
+
+    def spy(*a):
+        dts.append(orig(*a))
+        return dts[-1]
+    vo.adapt_dt = spy
+    try:
+        st.main_loop(4, case.dist, case.basis, case.elliptic, case.grids, case.flux, courant=0.209, acc=case.acc)
+    finally:
+        vo.adapt_dt = orig
+    assert len(dts) == 5 and st.steps_counter == 4
+    np.testing.assert_allclose(st.time_array[1:], np.cumsum(dts[1:]), rtol=1e-15)

@@ -23,6 +23,7 @@ EXPORTS = (
     "vl_convert_layout", "vl_resident_layout", "vl_last_error_string",
     "vl_ensemble_create", "vl_ensemble_destroy", "vl_ensemble_bind", "vl_ensemble_set_schedule",
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
+    "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -85,6 +86,9 @@ def load(path=LIB_PATH):
         "vl_state_elems_layout": [vp, i32, ctypes.POINTER(sz)],
         "vl_convert_layout": [vp, vp, i32, vp, i32, vp],
         "vl_resident_layout": [vp, ctypes.POINTER(i32)],
+        "vl_set_cfl": [vp, f64, f64, f64, f64],
+        "vl_clock_init": [vp, vp, f64, vp, vp],
+        "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
         "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
         "vl_ensemble_destroy": [vp],
         "vl_ensemble_bind": [vp, i32, vp, vp, vp, vp, vp, f64],
@@ -204,6 +208,9 @@ class Geometry:
         v = ctypes.c_int()
         check(load().vl_resident_layout(self.handle, ctypes.byref(v)), "vl_resident_layout")
         return int(v.value)
+
+    def set_cfl(self, courant, dx, du, dv):
+        check(load().vl_set_cfl(self.handle, float(courant), float(dx), float(du), float(dv)), "vl_set_cfl")
 
     def get_tuning(self, key):
         v = ctypes.c_int()

@@ -205,7 +205,23 @@ struct StageArgs {
     const unsigned *seed;
     int rounding;
     double scale, inv_scale;       // power of two
+    // device time step (adaptive runs, the kernels' ADT variants): c2dt = c2 * dt_dev[0]
+    const double *dt_dev;
+    double c2;
 };
+
+// Stage coefficient of the time step: c2 * dt, dt from device memory in
+// adaptive runs (the same product the host forms for a fixed dt).  The
+// product is wave-uniform; readfirstlane keeps it in scalar registers like
+// the kernel-argument value it replaces.
+__device__ __forceinline__ void stage_dt_from_device(StageArgs &s) {
+    if (s.dt_dev) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, s.c2 * s.dt_dev[0]);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+        s.c2dt = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    }
+}
 
 // Map a hardware workgroup id to a logical tile so that consecutive logical
 // tiles run on the same XCD (the hardware deals workgroup b to XCD b % 8).

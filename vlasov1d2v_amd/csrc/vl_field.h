@@ -131,7 +131,19 @@ struct FieldArgs {
     double poisson_coef;
     double dx, length;
     double acc;
+    // adaptive time step (main/timestep.py:231-242): when dt_next is set, the
+    // inverse sums also write courant / (s0/dx + (max|acc E| + bv)/du + s2/dv)
+    // there, with s0 = max|u|, bv = |acc B max|v||, s2 = |acc |B max|u|||
+    double *dt_next;
+    double cfl_courant, cfl_s0, cfl_bv, cfl_s2, cfl_dx, cfl_du, cfl_dv;
 };
+
+// The reference's adaptive step from max|acc E| (main/timestep.py:231-232,
+// 235-242): the same divisions and left-to-right sum.
+__device__ __forceinline__ double cfl_dt(const FieldArgs &a, double max_acc_e) {
+    const double s1 = max_acc_e + a.cfl_bv;
+    return a.cfl_courant / ((a.cfl_s0 / a.cfl_dx) + (s1 / a.cfl_du) + (a.cfl_s2 / a.cfl_dv));
+}
 
 // Forward nodal DFT and spectral solve for kDftModes modes per workgroup.
 //   cd   = from_rho ? mean(rho) - rho : rho          (every workgroup forms it
@@ -284,7 +296,7 @@ __device__ __forceinline__ void dft_inverse_body(
         }
         __syncthreads();
     }
-    const bool want_red = energy || maxabs;
+    const bool want_red = energy || maxabs || a.dt_next;
     if (g == 0) {
         DD en;
         double mx = 0.0;
@@ -338,6 +350,7 @@ __device__ __forceinline__ void dft_inverse_body(
         }
         if (energy) energy[0] = 0.5 * s.value();
         if (maxabs) maxabs[0] = mx;
+        if (a.dt_next) a.dt_next[0] = cfl_dt(a, mx);
         atomicExch(ticket, 0u);
     }
 }
@@ -376,6 +389,40 @@ __global__ __launch_bounds__(kFieldBlock) void k_max_abs_acc_e(const double *__r
     for (int k = threadIdx.x; k < count; k += kFieldBlock) mx = fmax(mx, fabs(acc * E[k]));
     const double g = block_max(mx, sh);
     if (threadIdx.x == 0) out[0] = g;
+}
+
+// Clock of an adaptive run in the workspace: clock[0] = dt of the next step,
+// clock[1] = time, clock[2] = dt computed from the latest stage-3 field.
+// Initial dt from the set-up field (main/timestep.py:88-89): max |acc E|
+// over the ghosted field, then the CFL rule.
+__global__ __launch_bounds__(kFieldBlock) void k_clock_init(const double *__restrict__ E, int count,
+                                                            FieldArgs a, double time,
+                                                            double *__restrict__ clock) {
+    __shared__ double sh[kFieldBlock];
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < count; k += kFieldBlock) mx = fmax(mx, fabs(a.acc * E[k]));
+    const double g = block_max(mx, sh);
+    if (threadIdx.x == 0) {
+        const double dt = cfl_dt(a, g);
+        clock[0] = dt;
+        clock[1] = time;
+        clock[2] = dt;
+    }
+}
+
+// After a step (main/timestep.py:184 then 96): the adapted dt becomes the
+// step's dt and the time advances by it; out (may be null) = (time, dt).
+__global__ void k_clock_advance(double *__restrict__ clock, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double dt = clock[2];
+        const double t = clock[1] + dt;
+        clock[0] = dt;
+        clock[1] = t;
+        if (out) {
+            out[0] = t;
+            out[1] = dt;
+        }
+    }
 }
 
 // Fill a small buffer with zeros (used instead of a memset so that the call

@@ -165,7 +165,7 @@ struct StageLds {
     static_assert(BG || (o_uh % 8 == 0), "the FP64 moment scratch aliases uh");
 };
 
-template <int N, int MODE, int S, int L>
+template <int N, int MODE, int S, int L, bool ADT = false>
 __global__ __launch_bounds__(kBlock, 3) void k_stage(
     const typename Store<S>::E *__restrict__ fin,  // stage input (interior; ghosts too when !periodic_x)
     const typename Store<S>::E *__restrict__ fgh,  // array whose u/v ghost planes are used
@@ -182,6 +182,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_stage(
     __shared__ typename Store<S>::C acc_l[StageLds<N, S>::n_acc];
     __shared__ double bgs[StageLds<N, S>::n_bgs];
     __shared__ double red_own[StageLds<N, S>::n_red];
+    if constexpr (ADT) stage_dt_from_device(s);   // adaptive runs: dt from the device clock
     const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
     const int stage_tid = threadIdx.x;
 #include "vl_stage_body.inc"

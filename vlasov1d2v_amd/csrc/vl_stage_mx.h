@@ -106,7 +106,7 @@ __device__ __forceinline__ constexpr int mx_base(int g) {
     return g <= MxShape<N>::GR ? (4 * g - 1 < N - 3 ? 4 * g - 1 : N - 3) : -1;
 }
 
-template <int N, int MODE, int S>
+template <int N, int MODE, int S, bool ADT = false>
 __global__ __launch_bounds__(kBlock, 4) void k_stage_mx(
     const uint16_t *__restrict__ fin,   // stage input
     const uint16_t *__restrict__ fgh,   // array whose u/v ghost elements are used
@@ -117,6 +117,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_mx(
     const double *__restrict__ vnodes,  // (nv+2)*N
     StageArgs s, const OpTables *__restrict__ tabp) {
     static_assert(MODE == kModeStage1 || MODE == kModeStage, "stage modes only");
+    if constexpr (ADT) stage_dt_from_device(s);   // adaptive runs: dt from the device clock
     using SH = MxShape<N>;
     using F = MxFmt<S>;
     constexpr int TV = SH::TV, P = SH::P, NB = SH::NB, PB = SH::PB, SX = SH::SX, SU = SH::SU;

@@ -153,7 +153,7 @@ struct RsLoads {
     unsigned ntail;
 };
 
-template <int N, int MODE, int S>
+template <int N, int MODE, int S, bool ADT = false>
 __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const uint16_t *__restrict__ fin,   // stage input
     const uint16_t *__restrict__ fgh,   // array whose u/v ghost elements are used
@@ -164,6 +164,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const double *__restrict__ vnodes,  // (nv+2)*N
     StageArgs s, const OpTables *__restrict__ tabp) {
     static_assert(MODE == kModeStage1 || MODE == kModeStage, "stage modes only");
+    if constexpr (ADT) stage_dt_from_device(s);   // adaptive runs: dt from the device clock
     using SH = RsShape<N>;
     using F = MxFmt<S>;
     using PK = RsPack<S>;

@@ -47,8 +47,12 @@ struct vl_geom {
     std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
     double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
     void **stage_events;   // optional: 6 hipEvent_t recorded around the stage kernels
+    // adaptive time step (main/timestep.py:231-242): Courant number, cell
+    // sizes, max |u| and max |v| over the ghosted nodes
+    int cfl_set;
+    double cfl_courant, cfl_dx, cfl_du, cfl_dv, umax, vmax;
     // workspace layout (byte offsets)
-    size_t off_partials, off_rho, off_coef, off_enpart, off_mxpart, off_ticket, ws_bytes;
+    size_t off_partials, off_rho, off_coef, off_enpart, off_mxpart, off_ticket, off_clock, ws_bytes;
     int inv_blocks;
 };
 
@@ -122,6 +126,7 @@ StageArgs make_stage_args(const vl_geom *h, int layout = kRefLayout) {
 
 FieldArgs make_field_args(const vl_geom *h, int from_rho) {
     FieldArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.nx = h->nx;
     a.n = h->n;
     a.M = h->M;
@@ -133,12 +138,36 @@ FieldArgs make_field_args(const vl_geom *h, int from_rho) {
     return a;
 }
 
+// CFL constants of the adaptive step, formed as main/timestep.py:235-242 forms
+// the speeds: s0 = max|u|, bv = |acc B max|v||, s2 = |acc |B max|u|||.
+void set_cfl_args(const vl_geom *h, FieldArgs &a) {
+    a.cfl_courant = h->cfl_courant;
+    a.cfl_s0 = std::fabs(h->umax);
+    a.cfl_bv = std::fabs(h->acc * h->bfield * h->vmax);
+    a.cfl_s2 = std::fabs(h->acc * std::fabs(h->bfield * h->umax));
+    a.cfl_dx = h->cfl_dx;
+    a.cfl_du = h->cfl_du;
+    a.cfl_dv = h->cfl_dv;
+}
+
 // ---- launch helpers (template dispatch over n) --------------------------
 
+// Stage kernels read the time step from kernel arguments; adaptive runs
+// (s.dt_dev set) launch the ADT variants, built for the resident layouts
+// only: element-major FP64/FP32 here, the 16-bit kernels below.
 template <int N, int MODE, int S, int L>
 hipError_t launch_stage_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                            const double *E, const StageArgs &s, hipStream_t st) {
     using T = typename Store<S>::E;
+    if (s.dt_dev) {
+        if constexpr (L == kElemLayout && MODE != kModeRhs && (S == VL_F64 || S == VL_F32)) {
+            hipLaunchKernelGGL((k_stage<N, MODE, S, L, true>), dim3(h->pencils), dim3(kBlock), 0, st,
+                               (const T *)fin, (const T *)fgh, (const T *)fn, (T *)fout, E, h->d_unodes,
+                               h->d_vnodes, s, h->d_tab);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     hipLaunchKernelGGL((k_stage<N, MODE, S, L>), dim3(h->pencils), dim3(kBlock), 0, st, (const T *)fin,
                        (const T *)fgh, (const T *)fn, (T *)fout, E, h->d_unodes, h->d_vnodes, s, h->d_tab);
     return hipGetLastError();
@@ -162,9 +191,14 @@ hipError_t launch_stage_ns(const vl_geom *h, const void *fin, const void *fgh, c
 template <int N, int MODE, int S>
 hipError_t launch_stage_mx_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                               const double *E, const StageArgs &s, hipStream_t st) {
-    hipLaunchKernelGGL((k_stage_mx<N, MODE, S>), dim3(h->pencils), dim3(kBlock), 0, st, (const uint16_t *)fin,
-                       (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
-                       h->d_vnodes, s, h->d_tab);
+    if (s.dt_dev)
+        hipLaunchKernelGGL((k_stage_mx<N, MODE, S, true>), dim3(h->pencils), dim3(kBlock), 0, st,
+                           (const uint16_t *)fin, (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E,
+                           h->d_unodes, h->d_vnodes, s, h->d_tab);
+    else
+        hipLaunchKernelGGL((k_stage_mx<N, MODE, S>), dim3(h->pencils), dim3(kBlock), 0, st, (const uint16_t *)fin,
+                           (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
+                           h->d_vnodes, s, h->d_tab);
     return hipGetLastError();
 }
 
@@ -177,9 +211,14 @@ hipError_t launch_stage_mx(const vl_geom *h, const void *fin, const void *fgh, c
 template <int N, int MODE, int S>
 hipError_t launch_stage_rs_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                               const double *E, const StageArgs &s, hipStream_t st) {
-    hipLaunchKernelGGL((k_stage_rs<N, MODE, S>), dim3(h->pencils4), dim3(kBlock), 0, st, (const uint16_t *)fin,
-                       (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
-                       h->d_vnodes, s, h->d_tab);
+    if (s.dt_dev)
+        hipLaunchKernelGGL((k_stage_rs<N, MODE, S, true>), dim3(h->pencils4), dim3(kBlock), 0, st,
+                           (const uint16_t *)fin, (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E,
+                           h->d_unodes, h->d_vnodes, s, h->d_tab);
+    else
+        hipLaunchKernelGGL((k_stage_rs<N, MODE, S>), dim3(h->pencils4), dim3(kBlock), 0, st, (const uint16_t *)fin,
+                           (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E, h->d_unodes,
+                           h->d_vnodes, s, h->d_tab);
     return hipGetLastError();
 }
 
@@ -293,6 +332,10 @@ unsigned int *ws_ticket(const vl_geom *h, void *ws) {
 }
 // rounding seed: the word after the ticket (zeroed with it by vl_workspace_init)
 unsigned int *ws_seed(const vl_geom *h, void *ws) { return ws_ticket(h, ws) + 1; }
+// clock of adaptive runs: (dt, time, next dt)
+double *ws_clock(const vl_geom *h, void *ws) {
+    return reinterpret_cast<double *>(static_cast<char *>(ws) + h->off_clock);
+}
 
 template <int N>
 hipError_t launch_reduce_n(const vl_geom *h, void *ws, double *rho, int tpe, unsigned *seed, hipStream_t st) {
@@ -325,8 +368,12 @@ hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st
 
 // spectral solve: forward DFT (M/16 workgroups), inverse sums (K/64 workgroups)
 hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho, void *ws, double *E,
-                        double *phi, double *energy, double *maxabs, hipStream_t st) {
-    const FieldArgs a = make_field_args(h, from_rho);
+                        double *phi, double *energy, double *maxabs, hipStream_t st, double *dt_next = nullptr) {
+    FieldArgs a = make_field_args(h, from_rho);
+    if (dt_next) {
+        set_cfl_args(h, a);
+        a.dt_next = dt_next;
+    }
     const int K = h->nx * h->n;
     const int gf = (h->M + kDftModes - 1) / kDftModes;
     hipLaunchKernelGGL(k_dft_forward, dim3(gf), dim3(kBlock), (size_t)K * sizeof(double), st, rho_or_cd,
@@ -341,10 +388,10 @@ hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho,
 
 // partials -> rho (workspace) -> E (+phi, energy)
 hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *phi, double *energy,
-                               hipStream_t st, int layout) {
+                               hipStream_t st, int layout, double *dt_next = nullptr) {
     hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st, layout, ws_seed(h, ws));
     if (e != hipSuccess) return e;
-    return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st);
+    return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st, dt_next);
 }
 
 hipError_t record_stage_event(const vl_geom *h, int idx, hipStream_t st) {
@@ -566,6 +613,8 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         h->tab.wu[i] = (float)(d->weights[i] / (d->ju * d->jv));
         h->tab.wv[i] = (float)d->weights[i];
     }
+    for (int i = 0; i < (d->nu + 2) * n; ++i) h->umax = std::max(h->umax, std::fabs(d->u_nodes[i]));
+    for (int i = 0; i < (d->nv + 2) * n; ++i) h->vmax = std::max(h->vmax, std::fabs(d->v_nodes[i]));
     h->TV = tile_tv(n);
     h->nchunk = (d->nv + h->TV - 1) / h->TV;
     h->tiles_per_ex = d->nu * h->nchunk;
@@ -626,6 +675,8 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     off = align_up(off + (size_t)h->inv_blocks * sizeof(double), 256);
     h->off_ticket = off;
     off = align_up(off + 256, 256);
+    h->off_clock = off;
+    off = align_up(off + 4 * sizeof(double), 256);
     h->ws_bytes = off;
 
     hipError_t e = hipGetDevice(&h->device);
@@ -761,26 +812,37 @@ int vl_swap_ghosts(vl_geom_t h, void *f, void *stream) {
     return swap_ghosts_layout(h, f, kRefLayout, stream);
 }
 
-int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double dt,
-                double *energy_out, void *ws, int flags, void *stream) {
+}  // extern "C"
+
+namespace {
+
+// One SSP-RK3 step; dt_dev = null: fixed dt, else the adaptive clock of the
+// workspace (dt read by the stage kernels, next dt from the stage-3 field).
+int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double dt, bool adaptive,
+             double *energy_out, double *clock_out, void *ws, int flags, void *stream, const char *what) {
     if (int r = check_handle(h)) return r;
-    if (!fv || !f1v || !f2v || !E || !ws) return fail(VL_E_NULL, "vl_rk3_step: null pointer");
+    if (!fv || !f1v || !f2v || !E || !ws) return fail(VL_E_NULL, "%s: null pointer", what);
     const size_t eb = storage_bytes(h->storage);
     if (!aligned(fv, eb) || !aligned(f1v, eb) || !aligned(f2v, eb) || !aligned(E, 8) || !aligned(ws, 256))
-        return fail(VL_E_ALIGN, "vl_rk3_step: misaligned pointer");
-    if (!std::isfinite(dt)) return fail(VL_E_NOTFINITE, "vl_rk3_step: dt not finite");
+        return fail(VL_E_ALIGN, "%s: misaligned pointer", what);
+    if (!std::isfinite(dt)) return fail(VL_E_NOTFINITE, "%s: dt not finite", what);
+    if (adaptive && !h->cfl_set) return fail(VL_E_ARG, "%s: adaptive steps need vl_set_cfl first", what);
+    if (adaptive && !(flags & (VL_STEP_ELEMENT_MAJOR | VL_STEP_ROW_SPLIT)))
+        return fail(VL_E_ARG, "%s: adaptive steps run in a resident layout (VL_STEP_ELEMENT_MAJOR / ROW_SPLIT)", what);
+    if (adaptive && (flags & VL_STEP_ELEMENT_MAJOR) && (h->storage == VL_BF16 || h->storage == VL_F16) && !h->mfma)
+        return fail(VL_E_ARG, "%s: adaptive 16-bit steps need the matrix-core kernels (VL_TUNE_MFMA 1)", what);
     if (h->storage != VL_F64 && !h->d_bg)
-        return fail(VL_E_ARG, "vl_rk3_step: reduced storage needs vl_set_background first");
+        return fail(VL_E_ARG, "%s: reduced storage needs vl_set_background first", what);
     void *f = fv, *f1 = f1v, *f2 = f2v;
     const int S = h->storage;
     if ((flags & VL_STEP_ELEMENT_MAJOR) && (flags & VL_STEP_ROW_SPLIT))
-        return fail(VL_E_ARG, "vl_rk3_step: VL_STEP_ELEMENT_MAJOR and VL_STEP_ROW_SPLIT are exclusive");
+        return fail(VL_E_ARG, "%s: VL_STEP_ELEMENT_MAJOR and VL_STEP_ROW_SPLIT are exclusive", what);
     const int LAY = (flags & VL_STEP_ROW_SPLIT) ? kRowSplitLayout
                     : (flags & VL_STEP_ELEMENT_MAJOR) ? kElemLayout : kRefLayout;
     if (LAY != kRefLayout && (!aligned(fv, 16) || !aligned(f1v, 16) || !aligned(f2v, 16)))
-        return fail(VL_E_ALIGN, "vl_rk3_step: element-wise arrays must be 16-byte aligned");
+        return fail(VL_E_ALIGN, "%s: element-wise arrays must be 16-byte aligned", what);
     if (LAY == kRowSplitLayout && ((h->storage != VL_BF16 && h->storage != VL_F16) || (h->n != 8 && h->n != 9)))
-        return fail(VL_E_ARG, "vl_rk3_step: the row-split layout needs BF16/FP16 storage and n = 8 or 9");
+        return fail(VL_E_ARG, "%s: the row-split layout needs BF16/FP16 storage and n = 8 or 9", what);
     hipStream_t st = (hipStream_t)stream;
     if (int r = swap_ghosts_layout(h, f, LAY, stream)) return r;
     StageArgs s = make_stage_args(h, LAY);
@@ -788,11 +850,14 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     s.write_partials = 1;
     s.partials = ws_partials(h, ws);
     s.seed = ws_seed(h, ws);
+    double *clock = ws_clock(h, ws);
+    if (adaptive) s.dt_dev = clock;
     if (!(flags & VL_STEP_RHO_VALID)) VL_HIP(launch_moment(h, S, LAY, f, s, st));
     // stage 1: f1 = f + dt L(f)                       (main/timestep.py:153-163)
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st, LAY));
     s.c0 = 0.0;
     s.c1 = 1.0;
+    s.c2 = 1.0;
     s.c2dt = dt;
     VL_HIP(record_stage_event(h, 0, st));
     VL_HIP(launch_stage<kModeStage1>(h, S, LAY, f, f, nullptr, f1, E, s, st));
@@ -801,19 +866,69 @@ int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *
     VL_HIP(field_from_partials(h, ws, E, nullptr, nullptr, st, LAY));
     s.c0 = 3.0 / 4.0;
     s.c1 = 1.0 / 4.0;
-    s.c2dt = (1.0 / 4.0) * dt;
+    s.c2 = 1.0 / 4.0;
+    s.c2dt = s.c2 * dt;
     VL_HIP(record_stage_event(h, 2, st));
     VL_HIP(launch_stage<kModeStage>(h, S, LAY, f1, f, f, f2, E, s, st));
     VL_HIP(record_stage_event(h, 3, st));
     // stage 3: f = 1/3 f + 2/3 f2 + 2/3 dt L(f2); E, phi, energy of f2's field
-    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st, LAY));
+    // (and, adaptive, the next dt from that field: main/timestep.py:184)
+    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st, LAY, adaptive ? clock + 2 : nullptr));
     s.c0 = 1.0 / 3.0;
     s.c1 = 2.0 / 3.0;
-    s.c2dt = (2.0 / 3.0) * dt;
+    s.c2 = 2.0 / 3.0;
+    s.c2dt = s.c2 * dt;
     VL_HIP(record_stage_event(h, 4, st));
     VL_HIP(launch_stage<kModeStage>(h, S, LAY, f2, f, f, f, E, s, st));
     VL_HIP(record_stage_event(h, 5, st));
+    if (adaptive) {
+        hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, st, clock, clock_out);
+        VL_HIP(hipGetLastError());
+    }
     return VL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vl_rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double dt,
+                double *energy_out, void *ws, int flags, void *stream) {
+    return rk3_step(h, fv, f1v, f2v, E, phi, dt, false, energy_out, nullptr, ws, flags, stream, "vl_rk3_step");
+}
+
+int vl_set_cfl(vl_geom_t h, double courant, double dx, double du, double dv) {
+    if (int r = check_handle(h)) return r;
+    if (!(courant > 0) || !(dx > 0) || !(du > 0) || !(dv > 0) || !std::isfinite(courant) || !std::isfinite(dx) ||
+        !std::isfinite(du) || !std::isfinite(dv))
+        return fail(VL_E_ARG, "vl_set_cfl: courant and cell sizes must be finite and > 0");
+    h->cfl_courant = courant;
+    h->cfl_dx = dx;
+    h->cfl_du = du;
+    h->cfl_dv = dv;
+    h->cfl_set = 1;
+    return VL_OK;
+}
+
+int vl_clock_init(vl_geom_t h, const double *E, double time, void *ws, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!E || !ws) return fail(VL_E_NULL, "vl_clock_init: null pointer");
+    if (!aligned(E, 8) || !aligned(ws, 256)) return fail(VL_E_ALIGN, "vl_clock_init: misaligned pointer");
+    if (!h->cfl_set) return fail(VL_E_ARG, "vl_clock_init: call vl_set_cfl first");
+    if (!std::isfinite(time)) return fail(VL_E_NOTFINITE, "vl_clock_init: time not finite");
+    FieldArgs a = make_field_args(h, 1);
+    set_cfl_args(h, a);
+    hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(kFieldBlock), 0, (hipStream_t)stream, E, (h->nx + 2) * h->n, a,
+                       time, ws_clock(h, ws));
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_rk3_step_adaptive(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double *energy_out,
+                         double *clock_out, void *ws, int flags, void *stream) {
+    if (clock_out && !aligned(clock_out, 8)) return fail(VL_E_ALIGN, "vl_rk3_step_adaptive: misaligned clock_out");
+    return rk3_step(h, fv, f1v, f2v, E, phi, 0.0, true, energy_out, clock_out, ws, flags, stream,
+                    "vl_rk3_step_adaptive");
 }
 
 int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems) {
@@ -1252,7 +1367,7 @@ int vl_ensemble_autotune(vl_ensemble_t e, int reps, float *table, int max_rows, 
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             cus = prop.multiProcessorCount;
     }
-    const int works[] = {4500, 9000, 18000, 36000, 72000};
+    const int works[] = {9000, 18000, 36000, 72000, 144000, 288000};
     const int grids[] = {0, 3 * cus, 6 * cus, 12 * cus};
     hipEvent_t t0, t1;
     VL_HIP(hipEventCreate(&t0));

@@ -10,6 +10,7 @@ same kernel statements; the compiler may fuse multiply-adds differently).
 The runs' potentials (``elliptic.potential``) are not updated.
 
     python -m vlasov1d2v_amd.ensemble [--runs 48] [--steps 200] [--storage f32] [--check-f64]
+                                      [--schedule work,grid,order]
 
 runs the seeded ensemble workload (48 runs; NX in {16, 32, 48, 64}, NU = NV
 in [24, 48], n in {6, 7, 8, 9}, ring index j in [1, 8], k0 in [0.05, 0.2],
@@ -74,8 +75,9 @@ def cfl_dt(run):
 class Ensemble:
     """Members stepped together; states stay resident (element-major) until ``finish()``.
 
-    ``dt``: one fixed step for every run, or None for 0.85 x each run's CFL
-    step at t = 0 (the margin the single-run workloads use below the rule).
+    ``dt``: one fixed step for every run, a sequence of one per run, or None
+    for 0.85 x each run's CFL step at t = 0 (the margin the single-run
+    workloads keep below the rule).
     """
 
     def __init__(self, members, storage="f32", device="cuda", dt=None, safety=0.85):
@@ -90,7 +92,12 @@ class Ensemble:
         for m in self.members:
             run = Run(m.n, m.nx, m.nu, m.nv, k0=m.k0, b_field=m.b_field, ring_j=m.ring_j, seed=m.seed,
                       dt=1.0, device=self.device, storage=storage)
-            step = float(dt) if dt is not None else safety * cfl_dt(run)
+            if dt is None:
+                step = safety * cfl_dt(run)
+            elif np.ndim(dt) == 0:
+                step = float(dt)
+            else:
+                step = float(dt[len(self.runs)])
             run.stepper.dt = run.stepper.fixed_dt = step
             self.runs.append(run)
             self.dts.append(step)
@@ -221,24 +228,71 @@ class EnsembleGraph:
 
 
 def time_steps(ens, steps, graph_steps):
-    """Seconds for ``steps`` ensemble steps replayed from graphs of ``graph_steps``."""
+    """Time ``steps`` ensemble steps replayed from one graph of ``graph_steps``
+    steps (after one eager step).  Returns (seconds of the replays, steps
+    replayed, energy series (1 + 1 + steps, runs) on the host: the initial
+    field, the eager step, the replayed steps)."""
     import torch
-    graph = ens.capture(graph_steps)
-    graph.replay()                                  # untimed
     reps = max(1, steps // graph_steps)
+    series = torch.empty((2 + reps * graph_steps, len(ens.runs)), dtype=torch.float64, device=ens.device)
+    series[0] = ens.energy0
+    ens.step(series[1])
+    graph = ens.capture(graph_steps)
     torch.cuda.synchronize(ens.device)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record()
-    for _ in range(reps):
+    for k in range(reps):
         graph.replay()
+        series[2 + k * graph_steps:2 + (k + 1) * graph_steps].copy_(graph.energies)
     t1.record()
     torch.cuda.synchronize(ens.device)
-    return t0.elapsed_time(t1) / 1e3, reps * graph_steps
+    return t0.elapsed_time(t1) / 1e3, reps * graph_steps, series.cpu().numpy()
+
+
+def run_workload(runs=48, seed=0, steps=200, graph_steps=20, storage="f32", autotune=True, check_f64=False,
+                 device="cuda:0", schedule=None):
+    """The seeded ensemble workload: build, autotune (or take ``schedule`` =
+    (work_per_item, grid, order)), time ``steps`` steps from graph replays;
+    optionally every member's error against FP64."""
+    import torch
+    members = draw_members(runs, seed)
+    dev = torch.device(device)
+    t0 = time.perf_counter()
+    ens = Ensemble(members, storage=storage, device=dev)
+    setup_s = time.perf_counter() - t0
+    if schedule is not None:
+        ens.set_schedule(*schedule)
+    sched = ens.autotune() if autotune and schedule is None else ens.schedule()
+    sec, done, series = time_steps(ens, steps, graph_steps)
+    out = {"metric": "ensemble RK3 DOF-updates/s", "runs": len(members), "seed": seed, "dof": ens.dof,
+           "storage": storage, "steps": done, "seconds": sec,
+           "dof_updates_per_s": ens.dof * done / sec, "ms_per_step": 1e3 * sec / done,
+           "schedule": sched, "setup_s": setup_s, "launches_per_step": 13,
+           "members": [dataclasses.asdict(m) for m in members]}
+    if ens.tuning:
+        out["autotune"] = ens.tuning
+    if check_f64:
+        # same members and dt, same step count (one eager step + graph replays)
+        total = ens.steps_done
+        ens.finish()
+        ref = Ensemble(members, storage="f64", device=dev, dt=ens.dts)
+        ref_series = ref.run(total, graph_steps=graph_steps)
+        ref.finish()
+        errs, eerrs = [], []
+        for r, (a, b) in enumerate(zip(ens.runs, ref.runs)):
+            fa = a.dist.total()[1:-1, :, 1:-1, :, 1:-1, :]
+            fb = b.dist.arr[1:-1, :, 1:-1, :, 1:-1, :]
+            errs.append(torch.linalg.vector_norm(fa - fb).item() / torch.linalg.vector_norm(fb).item())
+            eerrs.append(float(np.linalg.norm(series[:, r] - ref_series[:, r]) / np.linalg.norm(ref_series[:, r])))
+        out["rel_l2_f_vs_f64"] = {"max": max(errs), "median": float(np.median(errs)), "per_run": errs}
+        out["rel_l2_energy_vs_f64"] = {"max": max(eerrs), "median": float(np.median(eerrs)), "per_run": eerrs}
+        ref.close()
+    ens.close()
+    return out
 
 
 def main(argv=None):
-    import torch
     ap = argparse.ArgumentParser()
     ap.add_argument("--runs", type=int, default=48)
     ap.add_argument("--seed", type=int, default=0)
@@ -246,44 +300,13 @@ def main(argv=None):
     ap.add_argument("--graph-steps", type=int, default=20)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--schedule", default="", help="work_per_item,grid,order instead of autotuning")
     ap.add_argument("--check-f64", action="store_true",
                     help="also run every member in FP64 and report per-run errors")
     args = ap.parse_args(argv)
-    members = draw_members(args.runs, args.seed)
-    dev = torch.device("cuda:0")
-    t0 = time.perf_counter()
-    ens = Ensemble(members, storage=args.storage, device=dev)
-    setup_s = time.perf_counter() - t0
-    sched = ens.schedule() if args.no_autotune else ens.autotune()
-    ens.step()                                         # eager step before capture
-    torch.cuda.synchronize(dev)
-    sec, done = time_steps(ens, args.steps, args.graph_steps)
-    out = {"metric": "ensemble RK3 DOF-updates/s", "runs": len(members), "dof": ens.dof,
-           "storage": args.storage, "steps": done, "seconds": sec,
-           "dof_updates_per_s": ens.dof * done / sec, "ms_per_step": 1e3 * sec / done,
-           "schedule": sched, "setup_s": setup_s, "launches_per_step": 13,
-           "members": [dataclasses.asdict(m) for m in members]}
-    if ens.tuning:
-        out["autotune"] = ens.tuning
-    if args.check_f64:
-        # same members, same dt, same number of steps (1 eager + graph replays)
-        total = ens.steps_done
-        ens.finish()
-        ref = Ensemble(members, storage="f64", device=dev, dt=None)
-        for r64, r32, dt in zip(ref.runs, ens.runs, ens.dts):
-            assert abs(r64.stepper.dt - dt) <= 1e-15 * dt
-        ref.step()
-        if total > 1:
-            g = ref.capture(total - 1)
-            g.replay()
-        torch.cuda.synchronize(dev)
-        ref.finish()
-        errs = []
-        for a, b in zip(ens.runs, ref.runs):
-            fa = a.dist.total()[1:-1, :, 1:-1, :, 1:-1, :]
-            fb = b.dist.arr[1:-1, :, 1:-1, :, 1:-1, :]
-            errs.append(torch.linalg.vector_norm(fa - fb).item() / torch.linalg.vector_norm(fb).item())
-        out["rel_l2_f_vs_f64"] = {"max": max(errs), "median": float(np.median(errs)), "per_run": errs}
+    sched = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
+    out = run_workload(args.runs, args.seed, args.steps, args.graph_steps, args.storage,
+                       autotune=not args.no_autotune, check_f64=args.check_f64, schedule=sched)
     print(json.dumps(out), flush=True)
     return out
 

@@ -5,8 +5,11 @@
 launches, no host synchronisation).  ``nonlinear_ssp_rk_unfused`` composes
 the per-operator entry points in the reference's order and is kept for
 cross-checking.  With a fixed ``dt`` the loop never synchronises and can be
-captured in a HIP graph; with ``dt=None`` it follows the reference's CFL rule
-(main/timestep.py:184,231-242), which reads max|acc E| back every step.
+captured in a HIP graph.  With ``dt=None`` it follows the reference's CFL rule
+(main/timestep.py:184,231-242): ``main_loop`` evaluates the rule on the
+device after every step (``vl_rk3_step_adaptive``), so adaptive loops are
+graph-captured too and read the clock back once per graph replay; a single
+``nonlinear_ssp_rk`` call applies it on the host (one read of max|acc E|).
 """
 import gc
 import time as _time
@@ -129,6 +132,16 @@ class Stepper:
         the third stage's input (as in the reference) and, when given,
         ``energy_out`` (a 0-d/1-element GPU tensor) its field energy.
         """
+        if self.dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+        self._fused_step(distribution, elliptic, grids, dg_flux, energy_out)
+        if self.fixed_dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+
+    def _fused_step(self, distribution, elliptic, grids, dg_flux, energy_out=None, clock_out=None,
+                    adaptive=False):
+        """vl_rk3_step with self.dt, or (adaptive) vl_rk3_step_adaptive with
+        the workspace clock; clock_out: 2-element GPU tensor or None."""
         import torch
         arr = distribution.arr
         _check_state(arr, grids.shape, "nonlinear_ssp_rk", allow_reduced=True)
@@ -151,15 +164,17 @@ class Stepper:
         if res is not None:
             flags |= (_lib.VL_STEP_ROW_SPLIT if res["layout"] == _lib.VL_LAYOUT_ROW_SPLIT
                       else _lib.VL_STEP_ELEMENT_MAJOR)
-        if self.dt is None:
-            self._adapt(grids, elliptic, dg_flux, refs)
-        _lib.check(_lib.load().vl_rk3_step(
-            g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
-            _lib.ptr(elliptic.potential), float(self.dt), _lib.ptr(energy_out), _lib.ptr(ws), flags,
-            _lib.stream_ptr()), "vl_rk3_step")
+        if adaptive:
+            _lib.check(_lib.load().vl_rk3_step_adaptive(
+                g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
+                _lib.ptr(elliptic.potential), _lib.ptr(energy_out), _lib.ptr(clock_out), _lib.ptr(ws), flags,
+                _lib.stream_ptr()), "vl_rk3_step_adaptive")
+        else:
+            _lib.check(_lib.load().vl_rk3_step(
+                g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
+                _lib.ptr(elliptic.potential), float(self.dt), _lib.ptr(energy_out), _lib.ptr(ws), flags,
+                _lib.stream_ptr()), "vl_rk3_step")
         self._rho_token = (arr.data_ptr(), arr._version, ws.data_ptr())
-        if self.fixed_dt is None:
-            self._adapt(grids, elliptic, dg_flux, refs)
 
     def nonlinear_ssp_rk_unfused(self, distribution, basis, elliptic, grids, dg_flux, refs=None):
         """The same step composed from the per-operator GPU entry points in the
@@ -189,28 +204,37 @@ class Stepper:
 
     # -- time loop -------------------------------------------------------------
     def main_loop(self, distribution, basis, elliptic, grids, dg_flux, refs=None, save_file=None,
-                  steps=None, graph_steps=0):
+                  steps=None, graph_steps=0, device_dt=True):
         """Step until ``final_time`` (or exactly ``steps`` steps) and record the
         field-energy series as the reference does (main/timestep.py:81-139).
 
-        With a fixed dt the energies stay on the GPU until the loop ends, and
-        with ``graph_steps`` > 0 the loop replays a HIP graph of that many
-        steps (after one eager step; the remainder runs eagerly).
+        The energies (and, adaptive, the times) stay on the GPU until the loop
+        ends, and with ``graph_steps`` > 0 the loop replays a HIP graph of
+        that many steps (after one eager step; the remainder runs eagerly).
+        With ``dt=None`` the CFL rule runs on the device (``device_dt``);
+        ``device_dt=False`` keeps the host rule of ``nonlinear_ssp_rk``.
         """
         import torch
         t0 = _time.time()
         dev = distribution.arr.device
         e0 = elliptic.electric_energy(grids.x)
-        if self.fixed_dt is None:
+        if self.fixed_dt is None and device_dt and self.courant is not None \
+                and self._acc(dg_flux, refs) == dg_flux.acceleration_coefficient:
+            self._main_loop_device_dt(distribution, basis, elliptic, grids, dg_flux, refs, steps, graph_steps, e0)
+        elif self.fixed_dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
             self.field_energy = np.append(self.field_energy, float(e0.item()))
-            while (self.time < self.final_time) if steps is None else (self.steps_counter < steps):
+            self.resident_begin(distribution, grids, dg_flux, elliptic)
+            done = 0
+            while (self.time < self.final_time) if steps is None else (done < steps):
+                done += 1
                 energy = torch.empty((), dtype=torch.float64, device=dev)
                 self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs, energy_out=energy)
                 self.time += self.dt
                 self.steps_counter += 1
                 self.time_array = np.append(self.time_array, self.time)
                 self.field_energy = np.append(self.field_energy, float(energy.item()))
+            self.resident_end(distribution)
         else:
             if steps is None:
                 steps = int(np.ceil((self.final_time - self.time) / self.dt - 1e-9))
@@ -240,6 +264,69 @@ class Stepper:
             self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
         self.elapsed = _time.time() - t0
 
+    def _main_loop_device_dt(self, distribution, basis, elliptic, grids, dg_flux, refs, steps, graph_steps, e0):
+        """main/timestep.py:81-139 with the CFL rule on the device: the clock
+        (dt, time) lives in the workspace; the host reads it once per graph
+        replay (or per eager step) only to stop at ``final_time``.  A replay
+        of K steps is used only when even K steps of the largest possible dt
+        (max|acc E| = 0) stay below final_time, so the loop stops exactly
+        where the reference's does."""
+        import torch
+        dev = distribution.arr.device
+        self.field_energy = np.append(self.field_energy, float(e0.item()))
+        self.resident_begin(distribution, grids, dg_flux, elliptic)
+        g = self._res["geom"]
+        ws = g.workspace(dev)
+        g.set_cfl(self.courant, grids.x.dx, grids.u.dx, grids.v.dx)
+        if elliptic.electric_field is None or elliptic.electric_field.device != dev:
+            raise RuntimeError("main_loop: the set-up field must be solved first (main/main.py:131)")
+        if elliptic.potential is None or elliptic.potential.device != dev:
+            elliptic.potential = torch.zeros((grids.x.res, grids.x.order), dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().vl_clock_init(g.handle, _lib.ptr(elliptic.electric_field), float(self.time),
+                                             _lib.ptr(ws), _lib.stream_ptr()), "vl_clock_init")
+        # largest possible dt: the rule with max|acc E| = 0 (an upper bound of every step's dt)
+        acc, b = dg_flux.acceleration_coefficient, elliptic.magnetic_field
+        s0 = abs(grids.u.arr_max)
+        bv = abs(acc * b * grids.v.arr_max)
+        s2 = abs(acc * abs(b * grids.u.arr_max))
+        dt_max = self.courant / ((s0 / grids.x.dx) + (bv / grids.u.dx) + (s2 / grids.v.dx))
+        energies, clocks = [], []
+        done = 0
+        t_host = self.time
+        graph = None
+        K = int(graph_steps)
+        while (done < steps) if steps is not None else (t_host < self.final_time):
+            if steps is not None:
+                use_graph = K > 0 and done >= 1 and steps - done >= K
+            else:
+                use_graph = K > 0 and done >= 1 and t_host + K * dt_max < self.final_time
+            if use_graph:
+                if graph is None:
+                    graph = StepGraph(self, distribution, basis, elliptic, grids, dg_flux, K, refs, adaptive=True)
+                graph.replay()
+                energies.append(graph.energies.clone())
+                clocks.append(graph.clocks.clone())
+                done += K
+            else:
+                e = torch.empty(1, dtype=torch.float64, device=dev)
+                c = torch.empty((1, 2), dtype=torch.float64, device=dev)
+                self._fused_step(distribution, elliptic, grids, dg_flux, energy_out=e, clock_out=c[0], adaptive=True)
+                energies.append(e)
+                clocks.append(c)
+                done += 1
+            if steps is None:
+                t_host = float(clocks[-1][-1, 0].item())
+        self.resident_end(distribution)
+        torch.cuda.synchronize(dev)
+        if done:
+            en = torch.cat(energies).cpu().numpy()
+            ck = torch.cat(clocks).cpu().numpy()
+            self.field_energy = np.append(self.field_energy, en)
+            self.time_array = np.append(self.time_array, ck[:, 0])
+            self.time = float(ck[-1, 0])
+            self.dt = float(ck[-1, 1])
+        self.steps_counter += done
+
 
 class StepGraph:
     """``steps`` fixed-dt SSP-RK3 steps captured once in a HIP graph.
@@ -252,16 +339,17 @@ class StepGraph:
     ``steps`` steps and leaves the energies of those steps in ``energies``.
     """
 
-    def __init__(self, stepper, distribution, basis, elliptic, grids, dg_flux, steps, refs=None):
+    def __init__(self, stepper, distribution, basis, elliptic, grids, dg_flux, steps, refs=None, adaptive=False):
         import torch
-        if stepper.fixed_dt is None:
-            raise ValueError("a captured step needs a fixed dt")
+        if stepper.fixed_dt is None and not adaptive:
+            raise ValueError("a captured step needs a fixed dt, or the device clock (adaptive=True)")
         if stepper._rho_token is None:
             raise RuntimeError("step once eagerly before capturing (the first captured step reuses "
                                "the moment partials of the previous step)")
         dev = distribution.arr.device
         self.steps = int(steps)
         self.energies = torch.zeros(self.steps, dtype=torch.float64, device=dev)
+        self.clocks = torch.zeros((self.steps, 2), dtype=torch.float64, device=dev)   # (time, dt) per step
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(torch.cuda.current_stream(dev))
         self.graph = torch.cuda.CUDAGraph()
@@ -273,8 +361,12 @@ class StepGraph:
         try:
             with torch.cuda.graph(self.graph, stream=side, capture_error_mode="thread_local"):
                 for k in range(self.steps):
-                    stepper.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
-                                             energy_out=self.energies[k:k + 1])
+                    if adaptive:
+                        stepper._fused_step(distribution, elliptic, grids, dg_flux, energy_out=self.energies[k:k + 1],
+                                            clock_out=self.clocks[k], adaptive=True)
+                    else:
+                        stepper.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
+                                                 energy_out=self.energies[k:k + 1])
         finally:
             if gc_was_enabled:
                 gc.enable()
