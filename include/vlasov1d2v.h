@@ -281,6 +281,22 @@ int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream);
  * (work, grid, order, ms); *rows = rows measured.  Synchronises the stream. */
 int vl_ensemble_autotune(vl_ensemble_t e, int reps, float *table, int max_rows, int *rows, void *stream);
 
+/* The reference driver's initial condition (Distribution.initialize_gpu,
+ * main/grid.py:244-279 with eigenfunction, :207-242): f = ring + (when
+ * `perturbation`) the Bernstein-mode eigenfunction
+ *   2 delta_n Re(exp(i k1 x) (-i) exp(i beta sin phi) df_dv Upsilon / k1^2),
+ *   Upsilon = sum_{n=-terms..terms} n / (n - om) J_n(beta) exp(-i n phi),
+ *   beta = -k1 r om_pc,
+ * on every ghosted node of f (reference layout, FP64, vl_state_elems values),
+ * computed on the device (Bessel orders by backward recurrence).  om is
+ * complex (om_re + i om_im).  pert_only = 1 writes the perturbation alone
+ * (the stored deviation of the reduced-precision forms). */
+typedef struct vl_eigen_desc {
+    int32_t ring_j, terms, perturbation, pert_only;
+    double vt, k1, om_re, om_im, om_pc, delta_n;
+} vl_eigen_desc;
+int vl_init_eigen(vl_geom_t h, const vl_eigen_desc *desc, double *f, void *stream);
+
 /* Description of the last failure on this thread (static storage). */
 const char *vl_last_error_string(void);
 

@@ -21,9 +21,11 @@ the workload definition:
   Hesthaven & Warburton (``JacobiGL``-style, von Winckel's ``lglnodes``) in
   80-bit long double and rounded to FP64; they equal the literal tables of
   main/basis.py:9-57 to within a few ulp.
-* The initial condition is the ring distribution of main/grid.py:244-268
-  times a density modulation ``1 + a1 cos(k1 x) + sum_m a_m cos(m k1 x + phi_m)``
-  (the workload's definition) instead of the Bernstein eigenfunction.
+* The workloads' initial condition is the ring distribution of
+  main/grid.py:244-268 times a density modulation
+  ``1 + a1 cos(k1 x) + sum_m a_m cos(m k1 x + phi_m)`` (the workload's
+  definition); the reference's Bernstein eigenfunction (main/grid.py:207-242)
+  is ``eigenfunction`` / ``Distribution.initialize_eigen``.
 
 Pinning (tests/test_oracle_*.py):
 
@@ -47,6 +49,9 @@ Pinning (tests/test_oracle_*.py):
 * ``Stepper``          -- SSP-RK3 amplification polynomial on a linear
                           problem, Shu-Osher table.
 * ``electric_energy``  -- closed form for E = A cos(kx).
+* ``eigenfunction``    -- the Jacobi-Anger sum exp(-i b sin t) = sum_n J_n(b)
+                          exp(-i n t) at om = 0, and independent Bessel values
+                          (series) for the scipy ``jv`` it calls.
 * ``get_max_speeds`` / ``adapt_dt`` -- the rule's value at the 2.1M-DOF
                           set-up (0.00944, computed independently from the
                           grid and field) and the zero-field closed form; the
@@ -234,6 +239,29 @@ def ring_distribution(grids, vt, ring_j):
     return factor * np.tensordot(ix, pol_eig, axes=0)
 
 
+def eigenfunction(grids, vt, ring_j, om, om_pc, delta_n, terms=20):
+    """Bernstein-mode perturbation of the ring (main/grid.py:207-242, the
+    non-symmetric branch): on every ghosted node
+        2 delta_n Re( exp(i k1 x) * (-i) exp(i beta sin phi) * df_dv * Upsilon / k1^2 ),
+    Upsilon = sum_{n=-terms..terms} n / (n - om) J_n(beta) exp(-i n phi),
+    beta = -k1 r om_pc, phi = atan2(v, u), df_dv = f0 (j / (x + 1e-16) - 1) / vt^2,
+    f0 = x^j e^-x / (2 pi vt^2 j!), x = (r / vt)^2 / 2."""
+    u = np.tensordot(grids.u.arr, np.ones_like(grids.v.arr), axes=0)
+    v = np.tensordot(np.ones_like(grids.u.arr), grids.v.arr, axes=0)
+    r = np.sqrt(u ** 2.0 + v ** 2.0)
+    phi = np.arctan2(v, u)
+    k1 = grids.x.k1
+    beta = -k1 * r * om_pc
+    x = 0.5 * (r / vt) ** 2.0
+    f0 = 1 / (2.0 * np.pi * (vt ** 2.0) * math.factorial(ring_j)) * np.multiply(x ** ring_j, np.exp(-x))
+    df_dv = np.multiply(f0, (ring_j / (x + 1.0e-16) - 1.0)) / (vt ** 2.0)
+    upsilon = np.array([n / (n - om) * np.multiply(sp.jv(n, beta), np.exp(-1j * n * phi))
+                        for n in range(-terms, terms + 1)]).sum(axis=0)
+    eig = np.multiply(df_dv, upsilon) / k1 ** 2.0
+    vel_mode = -1j * np.multiply(np.exp(1j * np.multiply(beta, np.sin(phi))), eig)
+    return 2.0 * delta_n * np.real(np.tensordot(np.exp(1j * k1 * grids.x.arr), vel_mode, axes=0))
+
+
 def density_modulation(x, k1, amp1, amps, phases):
     """1 + amp1 cos(k1 x) + sum_m amps[m] cos((m+2) k1 x + phases[m])."""
     out = 1.0 + amp1 * np.cos(k1 * x)
@@ -258,6 +286,13 @@ class Distribution:
         ring = ring_distribution(grids, self.vt, self.ring_j)
         mod = density_modulation(grids.x.arr, grids.x.k1, amp1, amps, phases)
         self.arr = ring * mod[:, :, None, None, None, None]
+
+    def initialize_eigen(self, grids, om, om_pc, delta_n, perturbation=True):
+        """The reference's own initial condition (main/grid.py:244-279): the
+        ring plus, when ``perturbation``, the Bernstein eigenfunction."""
+        self.arr = ring_distribution(grids, self.vt, self.ring_j)
+        if perturbation:
+            self.arr = self.arr + eigenfunction(grids, self.vt, self.ring_j, om, om_pc, delta_n)
 
     def swap_ghost_cells_of_array(self):                    # main/grid.py:187-189
         self.arr[0, :, :, :, :, :] = np.copy(self.arr[-2, :, :, :, :, :])
@@ -466,7 +501,8 @@ class Case:
     """
 
     def __init__(self, n, nx, nu, nv, k0=0.1, vmax=8.5, B=0.1, ring_j=6, acc=-1.0,
-                 poisson=-1.0, vt=1.0, amp1=1e-3, amps=(1e-6, 1e-6, 1e-6), seed=0, phases=None):
+                 poisson=-1.0, vt=1.0, amp1=1e-3, amps=(1e-6, 1e-6, 1e-6), seed=0, phases=None,
+                 ic="modulated", om=0.9, om_pc=1.0, delta_n=0.01):
         self.n = n
         self.basis = Basis3D([n, n, n])
         L = 2.0 * np.pi / k0
@@ -478,7 +514,10 @@ class Case:
         self.phases = np.asarray(phases, dtype=np.float64)
         self.dist = Distribution(vt, ring_j, [nx, nu, nv], [n, n, n])
         self.dist.initialize_quad_weights(self.grids)
-        self.dist.initialize(self.grids, amp1, amps, self.phases)
+        if ic == "eigen":                                    # main/main.py:73-77
+            self.dist.initialize_eigen(self.grids, om, om_pc, delta_n)
+        else:
+            self.dist.initialize(self.grids, amp1, amps, self.phases)
         self.elliptic = Elliptic(poisson)
         self.elliptic.set_magnetic_field(B)
         n0 = self.dist.moment_zero()

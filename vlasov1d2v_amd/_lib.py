@@ -23,7 +23,7 @@ EXPORTS = (
     "vl_convert_layout", "vl_resident_layout", "vl_last_error_string",
     "vl_ensemble_create", "vl_ensemble_destroy", "vl_ensemble_bind", "vl_ensemble_set_schedule",
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
-    "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive",
+    "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -44,6 +44,15 @@ class VlGeomDesc(ctypes.Structure):
         ("up", ctypes.c_void_p), ("xi", ctypes.c_void_p), ("weights", ctypes.c_void_p),
         ("x_nodes", ctypes.c_void_p), ("u_nodes", ctypes.c_void_p), ("v_nodes", ctypes.c_void_p),
         ("wave_numbers", ctypes.c_void_p), ("transform", ctypes.c_void_p), ("phases", ctypes.c_void_p),
+    ]
+
+
+class VlEigenDesc(ctypes.Structure):
+    _fields_ = [
+        ("ring_j", ctypes.c_int32), ("terms", ctypes.c_int32), ("perturbation", ctypes.c_int32),
+        ("pert_only", ctypes.c_int32), ("vt", ctypes.c_double), ("k1", ctypes.c_double),
+        ("om_re", ctypes.c_double), ("om_im", ctypes.c_double), ("om_pc", ctypes.c_double),
+        ("delta_n", ctypes.c_double),
     ]
 
 
@@ -86,6 +95,7 @@ def load(path=LIB_PATH):
         "vl_state_elems_layout": [vp, i32, ctypes.POINTER(sz)],
         "vl_convert_layout": [vp, vp, i32, vp, i32, vp],
         "vl_resident_layout": [vp, ctypes.POINTER(i32)],
+        "vl_init_eigen": [vp, ctypes.POINTER(VlEigenDesc), vp, vp],
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],

@@ -18,6 +18,7 @@
 #include "vl_stage_mx.h"
 #include "vl_stage_rs.h"
 #include "vl_ensemble.h"
+#include "vl_init.h"
 
 #include <vector>
 
@@ -38,7 +39,7 @@ struct vl_geom {
     int64_t sx, sxe, su, sue;
     size_t state_elems;
     int device;
-    double *d_unodes, *d_vnodes, *d_wts, *d_kwave;
+    double *d_xnodes, *d_unodes, *d_vnodes, *d_wts, *d_kwave;
     double2 *d_T, *d_phases;
     double *d_bg;          // reduced forms: G+, G-, Hv background tables
     float *d_bg32;         // reduced forms: scaled operator images of f0 (row-split kernel)
@@ -686,6 +687,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         const std::vector<unsigned> fr = build_mxfrag(h->tab, n);
         e = alloc_copy(&h->d_mxfrag, fr.data(), fr.size());
     }
+    if (e == hipSuccess && d->x_nodes) e = alloc_copy(&h->d_xnodes, d->x_nodes, (size_t)(d->nx + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_unodes, d->u_nodes, (size_t)(d->nu + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_vnodes, d->v_nodes, (size_t)(d->nv + 2) * n);
     if (e == hipSuccess) e = alloc_copy(&h->d_wts, d->weights, (size_t)n);
@@ -704,8 +706,8 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
 int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
-    void *bufs[] = {h->d_unodes, h->d_vnodes, h->d_wts,   h->d_kwave, h->d_T,
-                    h->d_phases, h->d_bg,     h->d_bg32, h->d_tab, h->d_mxfrag};
+    void *bufs[] = {h->d_xnodes, h->d_unodes, h->d_vnodes, h->d_wts,  h->d_kwave, h->d_T,
+                    h->d_phases, h->d_bg,     h->d_bg32,   h->d_tab, h->d_mxfrag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -1043,6 +1045,45 @@ int vl_resident_layout(vl_geom_t h, int *layout) {
 int vl_set_stage_events(vl_geom_t h, void **events) {
     if (int r = check_handle(h)) return r;
     h->stage_events = events;
+    return VL_OK;
+}
+
+int vl_init_eigen(vl_geom_t h, const vl_eigen_desc *d, double *f, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!d || !f) return fail(VL_E_NULL, "vl_init_eigen: null pointer");
+    if (!aligned(f, 8)) return fail(VL_E_ALIGN, "vl_init_eigen: misaligned f");
+    if (!h->d_xnodes) return fail(VL_E_ARG, "vl_init_eigen: the geometry was created without x nodes");
+    if (d->ring_j < 0 || d->ring_j > 170) return fail(VL_E_ARG, "vl_init_eigen: ring_j=%d outside [0,170]", d->ring_j);
+    if (d->terms < 0 || d->terms > kEigenMaxTerms)
+        return fail(VL_E_ARG, "vl_init_eigen: terms=%d outside [0,%d]", d->terms, kEigenMaxTerms);
+    if (!(d->vt > 0) || !std::isfinite(d->k1) || !std::isfinite(d->om_re) || !std::isfinite(d->om_im) ||
+        !std::isfinite(d->om_pc) || !std::isfinite(d->delta_n) || (d->perturbation && d->k1 == 0.0))
+        return fail(VL_E_ARG, "vl_init_eigen: vt > 0 and finite parameters (k1 != 0) required");
+    EigenArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = h->nx;
+    a.nu = h->nu;
+    a.nv = h->nv;
+    a.n = h->n;
+    a.ring_j = d->ring_j;
+    a.terms = d->terms;
+    a.perturbation = d->perturbation ? 1 : 0;
+    a.pert_only = d->pert_only ? 1 : 0;
+    a.vt = d->vt;
+    a.k1 = d->k1;
+    a.om_re = d->om_re;
+    a.om_im = d->om_im;
+    a.om_pc = d->om_pc;
+    a.delta_n = d->delta_n;
+    double jfact = 1.0;
+    for (int k = 2; k <= d->ring_j; ++k) jfact *= (double)k;
+    // main/grid.py:253 and :213, in the reference's operation order
+    a.ring_factor = 1.0 / ((std::pow(2.0, d->ring_j)) * (2.0 * M_PI) * (d->vt * d->vt) * jfact);
+    a.f0_factor = 1.0 / (2.0 * M_PI * (d->vt * d->vt) * jfact);
+    const int nvel = (h->nu + 2) * h->n * (h->nv + 2) * h->n;
+    hipLaunchKernelGGL(k_init_eigen<kEigenMaxTerms>, dim3((nvel + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       (hipStream_t)stream, f, h->d_xnodes, h->d_unodes, h->d_vnodes, a);
+    VL_HIP(hipGetLastError());
     return VL_OK;
 }
 

@@ -6,6 +6,7 @@ The distribution function lives on the GPU as a torch tensor of the
 reference's ghosted layout (NX+2, n, NU+2, n, NV+2, n); its moment and ghost
 exchange run in the HIP library.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -196,20 +197,42 @@ class Distribution:
         return factor * pol                                   # (NU+2, n, NV+2, n)
 
     def initialize_gpu(self, grids, modulation=None):
-        """Set f = ring(u, v) * m(x) on every ghosted node and upload it.
+        """Initial state on every ghosted node.
 
-        ``modulation`` is an (NX+2, n) host array of the x-dependent factor
-        (default: 1).  The reference's Bernstein-eigenfunction perturbation
-        (main/grid.py:201-242) is not part of this path.
+        Without ``modulation`` this is the reference's own initial condition
+        (main/grid.py:244-279): the ring plus, when ``perturbation``, the
+        Bernstein-mode eigenfunction with this distribution's ``om``,
+        ``om_pc`` and ``delta_n`` (main/grid.py:207-242), computed on the
+        device (``vl_init_eigen``).  With ``modulation``, an (NX+2, n) host
+        array, it is ring(u, v) * modulation(x) (the workloads' condition).
         """
         import torch
         self._grids = grids
+        if modulation is None:
+            self.arr = self._eigen_state(grids, pert_only=False)
+            self._ic = "eigen"
+            if self.quad_weights is None:
+                self.initialize_quad_weights(grids)
+            return
         ring = torch.from_numpy(self.ring(grids)).to(self.device)
         mod = np.ones_like(grids.x.arr) if modulation is None else np.asarray(modulation)
         mod = torch.from_numpy(np.ascontiguousarray(mod)).to(self.device)
         self.arr = (ring[None, None, :, :, :, :] * mod[:, :, None, None, None, None]).contiguous()
         if self.quad_weights is None:
             self.initialize_quad_weights(grids)
+
+    def _eigen_state(self, grids, pert_only):
+        """ring + eigenfunction (or the eigenfunction alone) as an FP64 GPU tensor."""
+        import torch
+        g = grids.geometry()
+        om = complex(self.om)
+        d = _lib.VlEigenDesc(ring_j=int(self.ring_j), terms=20, perturbation=1 if self.perturbation else 0,
+                             pert_only=1 if pert_only else 0, vt=float(self.vt), k1=float(grids.x.k1),
+                             om_re=om.real, om_im=om.imag, om_pc=float(self.om_pc), delta_n=float(self.delta_n))
+        out = torch.empty(grids.shape, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().vl_init_eigen(g.handle, ctypes.byref(d), _lib.ptr(out), _lib.stream_ptr()),
+                   "vl_init_eigen")
+        return out
 
     def initialize_ring_modulated(self, grids, amp1, amps, phases):
         """f = ring(u,v) * (1 + amp1 cos(k1 x) + sum_m amps[m] cos((m+2) k1 x + phases[m]))."""
@@ -221,6 +244,7 @@ class Distribution:
             mod = mod + a * np.cos((m + 2) * k1 * x + ph)
             pert = pert + a * np.cos((m + 2) * k1 * x + ph)
         self._pert = pert
+        self._ic = "modulated"
         self.initialize_gpu(grids, modulation=mod)
 
     def to_storage(self, grids, storage):
@@ -231,15 +255,19 @@ class Distribution:
         import torch
         if storage == "f64":
             return
-        if getattr(self, "_pert", None) is None:
-            raise RuntimeError("to_storage needs initialize_ring_modulated first")
+        ic = getattr(self, "_ic", None)
+        if ic not in ("modulated", "eigen"):
+            raise RuntimeError("to_storage needs initialize_ring_modulated or initialize_gpu first")
         self.storage = storage
         self.scale = STORAGE_SCALE[storage]
         ring_h = self.ring(grids)
         self.f0 = ring_h
-        ring = torch.from_numpy(ring_h).to(self.device)
-        pert = torch.from_numpy(np.ascontiguousarray(self._pert)).to(self.device)
-        g = ring[None, None, :, :, :, :] * pert[:, :, None, None, None, None]
+        if ic == "eigen":
+            g = self._eigen_state(grids, pert_only=True)   # f - f0: the eigenfunction alone
+        else:
+            ring = torch.from_numpy(ring_h).to(self.device)
+            pert = torch.from_numpy(np.ascontiguousarray(self._pert)).to(self.device)
+            g = ring[None, None, :, :, :, :] * pert[:, :, None, None, None, None]
         self.arr = (g * self.scale).to(storage_dtype(storage)).contiguous()
         del g
         geo = grids.geometry(STORAGE_CODES[storage])

@@ -54,16 +54,25 @@ class Run:
 
     def __init__(self, n, nx, nu, nv, k0=0.1, b_field=0.1, ring_j=6, vmax=8.5, vt=1.0, acc=-1.0,
                  poisson=-1.0, amp1=1e-3, amps=(1e-6, 1e-6, 1e-6), seed=0, phases=None, dt=0.002,
-                 device="cuda", storage="f64"):
+                 device="cuda", storage="f64", ic="modulated", om=0.9, om_pc=1.0, delta_n=0.01):
+        """``ic="modulated"``: the workloads' ring x density modulation;
+        ``ic="eigen"``: the reference driver's ring + Bernstein eigenfunction
+        (main/main.py:73-77) with ``om``, ``om_pc``, ``delta_n``."""
         self.basis = Basis3D(np.array([n, n, n]))
         L = 2.0 * np.pi / k0
         lows = np.array([-L / 2.0, -vmax * vt, -vmax * vt])
         highs = np.array([L / 2.0, vmax * vt, vmax * vt])
         self.grids = Grid3D(self.basis, lows, highs, np.array([nx, nu, nv]))
         self.phases = seeded_phases(seed, len(amps)) if phases is None else np.asarray(phases)
-        self.dist = Distribution(vt, ring_j, [nx, nu, nv], [n, n, n], device=device)
+        self.dist = Distribution(vt, ring_j, [nx, nu, nv], [n, n, n], om=om, om_pc=om_pc, delta_n=delta_n,
+                                 device=device)
         self.dist.initialize_quad_weights(self.grids)
-        self.dist.initialize_ring_modulated(self.grids, amp1, amps, self.phases)
+        if ic == "eigen":
+            self.dist.initialize_gpu(self.grids)
+        elif ic == "modulated":
+            self.dist.initialize_ring_modulated(self.grids, amp1, amps, self.phases)
+        else:
+            raise ValueError(f"unknown initial condition {ic!r}")
         self.elliptic = Elliptic(poisson)
         self.elliptic.set_magnetic_field(b_field)
         self.flux = DGFlux(np.array([nx + 2, nu + 2, nv + 2]), np.array([n, n, n]), acc)
