@@ -281,6 +281,21 @@ int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream);
  * (work, grid, order, ms); *rows = rows measured.  Synchronises the stream. */
 int vl_ensemble_autotune(vl_ensemble_t e, int reps, float *table, int max_rows, int *rows, void *stream);
 
+/* Stabilized central-flux DG Poisson solve (Elliptic.poisson,
+ * main/elliptic.py:86-115), the reference's alternative to poisson2.  The
+ * operator is assembled and inverted on the host as the reference does
+ * (build_central_flux_operator / invert, main/elliptic.py:24-84) and
+ * uploaded once: inv_op ((nx n + 1)^2, row-major, gauge row/column
+ * included), grad_op (the gradient operator as an (nx n) x (nx n) row-major
+ * matrix indexed [(i, j), (e, k)]) and the n x n mass matrix.  Allocates its
+ * device tables and scratch on the first call.
+ * vl_poisson_central: E (nx+2)*n with periodic ghosts and phi nx*n (may be
+ * NULL) from the charge density cd (nx*n); anti_alias = 1 re-sums phi and E
+ * through the nodal Fourier transform (main/elliptic.py:101-113). */
+int vl_set_central_flux(vl_geom_t h, const double *inv_op, const double *grad_op, const double *mass);
+int vl_poisson_central(vl_geom_t h, const double *charge_density, double *E, double *phi, int anti_alias,
+                       void *stream);
+
 /* The reference driver's initial condition (Distribution.initialize_gpu,
  * main/grid.py:244-279 with eigenfunction, :207-242): f = ring + (when
  * `perturbation`) the Bernstein-mode eigenfunction

@@ -52,6 +52,14 @@ Pinning (tests/test_oracle_*.py):
 * ``eigenfunction``    -- the Jacobi-Anger sum exp(-i b sin t) = sum_n J_n(b)
                           exp(-i n t) at om = 0, and independent Bessel values
                           (series) for the scipy ``jv`` it calls.
+* ``Basis1D.mass_matrix`` / ``advection_matrix`` / ``derivative_matrix`` --
+                          Gauss-Legendre quadrature of Lagrange polynomials,
+                          barycentric differentiation, adv = D^T W.
+* ``Elliptic.poisson`` (central flux) -- the exact solution of a single
+                          Fourier mode, the gauge (zero quadrature mean of the
+                          potential) and E = -d(phi)/dx of the discrete potential.
+* ``Stepper.ssp_rk_update`` -- the amplification polynomial of the linear
+                          SSP-RK3 (1 + z + z^2/2 + z^3/6).
 * ``get_max_speeds`` / ``adapt_dt`` -- the rule's value at the 2.1M-DOF
                           set-up (0.00944, computed independently from the
                           grid and field) and the zero-field closed form; the
@@ -126,6 +134,42 @@ class Basis1D:
         self.invm = self.inv_mass_matrix()
         self.up = self.internal_flux()
         self.xi = self.invm[:, np.array([0, -1])]          # main/basis.py:110
+        # central-flux Poisson operators (main/basis.py:98-116)
+        self.mass = self.mass_matrix()
+        self.face_mass = np.eye(self.order)[:, np.array([0, -1])]
+        self.adv = self.advection_matrix()
+        self.stf = self.adv.T
+        self.np_xi = self.xi
+        self.der = self.derivative_matrix()
+
+    def mass_matrix(self):                                  # main/basis.py:139-148
+        approx_mass = np.diag(self.weights)
+        p = sp.legendre(self.order - 1)
+        v = np.multiply(self.weights, p(self.nodes))
+        a = -self.order * (self.order - 1) / (2 * (2 * self.order - 1))
+        return approx_mass + a * np.outer(v, v)
+
+    def advection_matrix(self):                             # main/basis.py:150-163
+        n = self.order
+        adv = np.zeros((n, n))
+        for i in range(n):
+            for j in range(n):
+                adv[i, j] = self.weights[i] * self.weights[j] * sum(
+                    self.eigenvalues[s] * sp.legendre(s)(self.nodes[i]) *
+                    sp.legendre(s).deriv()(self.nodes[j]) for s in range(n))
+        adv[np.abs(adv) < 1.0e-15] = 0
+        return adv
+
+    def derivative_matrix(self):                            # main/basis.py:190-202
+        n = self.order
+        der = np.zeros((n, n))
+        for i in range(n):
+            for j in range(n):
+                der[i, j] = self.weights[j] * sum(
+                    self.eigenvalues[s] * sp.legendre(s).deriv()(self.nodes[i]) *
+                    sp.legendre(s)(self.nodes[j]) for s in range(n))
+        der[np.abs(der) < 1.0e-15] = 0
+        return der
 
     def set_eigenvalues(self):                              # main/basis.py:126-132
         ev = np.array([(2.0 * s + 1) / 2.0 for s in range(self.order - 1)])
@@ -331,6 +375,59 @@ class Elliptic:
         self.electric_field[0, :] = self.electric_field[-2, :]
         self.electric_field[-1, :] = self.electric_field[1, :]
 
+    def build_central_flux_operator(self, grid, basis):     # main/elliptic.py:24-81
+        """The stabilized central-flux DG Laplacian, assembled column by column
+        from nodal indicators as the reference does, with the gauge row and
+        column (zero quadrature mean of the potential)."""
+        res, order = grid.res, grid.order
+        indicator = np.zeros((res, order))
+        face_diff0 = np.zeros((res, 2))
+        face_diff1 = np.zeros((res, 2))
+        num_flux = np.zeros_like(face_diff0)
+        grad_num_flux = np.zeros_like(face_diff1)
+        op = np.zeros((res, order, res, order))
+        self.gradient_operator = np.zeros_like(op)
+        for i in range(res):
+            for j in range(order):
+                indicator[i, j] = 1.0
+                face_diff0[:, 0] = indicator[:, 0] - np.roll(indicator[:, -1], 1)
+                face_diff0[:, 1] = indicator[:, -1] - np.roll(indicator[:, 0], -1)
+                num_flux[:, 0] = 0.5 * face_diff0[:, 0]
+                num_flux[:, 1] = -0.5 * face_diff0[:, 1]
+                grad = (np.tensordot(basis.der, indicator, axes=([1], [1])) +
+                        np.tensordot(basis.np_xi, num_flux, axes=([1], [1]))).T
+                face_diff1[:, 0] = grad[:, 0] - np.roll(grad[:, -1], 1)
+                face_diff1[:, 1] = grad[:, -1] - np.roll(grad[:, 0], -1)
+                grad_num_flux[:, 0] = 0.5 * face_diff1[:, 0]
+                grad_num_flux[:, 1] = -0.5 * face_diff1[:, 1]
+                operator = (np.tensordot(basis.stf, grad, axes=([1], [1])) +
+                            np.tensordot(basis.face_mass, grad_num_flux + face_diff0, axes=([1], [1]))).T
+                op[i, j, :, :] = operator
+                self.gradient_operator[i, j, :, :] = grad
+                indicator[i, j] = 0
+        qw = grid.quad_weights.reshape(res * order, 1)
+        op0 = np.hstack([op.reshape(res * order, res * order), qw])
+        self.central_flux_operator = np.vstack([op0, np.append(grid.quad_weights.flatten(), 0)])
+        self.central_flux_operator[np.abs(self.central_flux_operator) < 1.0e-15] = 0
+
+    def invert(self):                                       # main/elliptic.py:83-84
+        self.inv_op = np.linalg.inv(self.central_flux_operator)
+
+    def poisson(self, charge_density, grid, basis, anti_alias=True):   # main/elliptic.py:86-115
+        rhs = np.zeros((grid.res * grid.order + 1))
+        rhs[:-1] = self.poisson_coefficient * np.tensordot(charge_density, basis.mass, axes=([1], [1])).flatten()
+        sol = np.matmul(self.inv_op, rhs)[:-1] / (grid.J ** 2.0)
+        self.potential = sol.reshape(grid.res, grid.order)
+        if anti_alias:
+            self.potential = grid.sum_fourier(grid.fourier_basis(self.potential))
+        self.electric_field = np.zeros_like(grid.arr)
+        self.electric_field[1:-1, :] = -1.0 * (grid.J * np.tensordot(self.gradient_operator, self.potential,
+                                                                     axes=([0, 1], [0, 1])))
+        if anti_alias:
+            self.electric_field[1:-1, :] = grid.sum_fourier(grid.fourier_basis(self.electric_field[1:-1, :]))
+        self.electric_field[0, :] = self.electric_field[-2, :]
+        self.electric_field[-1, :] = self.electric_field[1, :]
+
     def set_magnetic_field(self, magnetic_field):           # main/elliptic.py:147-148
         self.magnetic_field = magnetic_field
 
@@ -424,6 +521,14 @@ def get_max_speeds(grids, elliptic, acc):                   # main/timestep.py:2
         acc * abs(elliptic.magnetic_field * grids.u.arr_max)]))
 
 
+# Linear SSP-RK coefficients (main/timestep.py:11-21)
+SSP_RK_LINEAR = {
+    1: [1],
+    2: [1 / 2, 1 / 2],
+    3: [1 / 3, 1 / 2, 1 / 6],
+    4: [3 / 8, 1 / 3, 1 / 4, 1 / 24],
+}
+
 # Courant numbers of the 3-stage row, n = 1..9 (main/timestep.py:24-33)
 COURANT_NUMBERS_RK3 = [1.256, 0.409, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033]
 
@@ -469,6 +574,28 @@ class Stepper:
             c = self.coefficients[i - 1]
             arr_stages[g_i] = c[0] * distribution.arr[ng] + c[1] * arr_stages[g_i1] + c[2] * self.dt * df_dt
         distribution.arr[ng] = arr_stages[-1, 1:-1, :, 1:-1, :, 1:-1, :]
+
+    def ssp_rk_update(self, distribution, basis, elliptic, grids, dg_flux, coefficients=None):
+        """The linear SSP-RK update (main/timestep.py:189-229) with the
+        central-flux Poisson solve (elliptic.poisson); the reference's
+        closing adapt_time_step is left to the caller."""
+        c = SSP_RK_LINEAR[self.time_order] if coefficients is None else coefficients
+        ng = grids.no_ghost_slice
+        distribution.swap_ghost_cells_of_array()
+        copy = np.copy(distribution.arr)
+        distribution.arr[ng] = c[0] * distribution.arr[ng]
+
+        def field_of(arr):
+            arr[0] = np.copy(arr[-2])
+            arr[-1] = np.copy(arr[1])
+            density = distribution.moment_zero_of_arr(arr)
+            elliptic.poisson(np.mean(density) - density, grids.x, basis.b1)
+        for i in range(1, len(c)):
+            field_of(copy)
+            copy[ng] += self.dt * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
+            distribution.arr[ng] += c[i] * copy[ng]
+        field_of(copy)
+        distribution.arr[ng] += c[-1] * self.dt * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
 
     def main_loop(self, steps, distribution, basis, elliptic, grids, dg_flux, courant=None, acc=None,
                   final_time=None):  # main/timestep.py:81-139

@@ -103,3 +103,20 @@ def test_vandermonde_inverse(n):
     # the reference's V^-1 uses the Lobatto eigenvalue (n-1)/2 in the last row,
     # which makes it the exact inverse of V
     assert np.max(np.abs(b.vandermonde_inverse @ V - np.eye(n))) < 1e-12
+
+
+@pytest.mark.parametrize("n", NS)
+def test_mass_advection_and_derivative_matrices(n):
+    """mass = int l_i l_j (Gauss-Legendre quadrature of the Lagrange basis),
+    der[i, j] = l_j'(x_i) (product rule), adv = der^T W (main/basis.py:139-163,190-202)."""
+    b = oracle.Basis1D(n)
+    xg, wg = npleg.leggauss(n + 2)
+    L = lagrange_matrix(b.nodes, xg)
+    M = (L * wg) @ L.T
+    assert np.max(np.abs(b.mass - M)) < 1e-14
+    D = lagrange_deriv_matrix(b.nodes, b.nodes).T        # D[i, j] = l_j'(x_i)
+    D[np.abs(D) < 1e-15] = 0
+    assert np.max(np.abs(b.der - D)) < 1e-12 * max(1.0, np.max(np.abs(D)))
+    assert np.max(np.abs(b.adv - (D.T * b.weights[None, :]))) < 1e-12 * max(1.0, np.max(np.abs(D)))
+    assert np.array_equal(b.stf, b.adv.T)
+    assert np.array_equal(b.face_mass, np.eye(n)[:, [0, -1]])

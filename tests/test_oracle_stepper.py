@@ -124,3 +124,21 @@ def test_adaptive_loop_advances_time_by_the_new_dt():
         vo.adapt_dt = orig
     assert len(dts) == 5 and st.steps_counter == 4
     np.testing.assert_allclose(st.time_array[1:], np.cumsum(dts[1:]), rtol=1e-15)
+
+
+@pytest.mark.parametrize("z", [-0.5, 0.3, -2.0, 1e-3])
+def test_linear_ssp_rk3_amplification_polynomial(z):
+    """main/timestep.py:189-229 with L f = lam f: the interior is multiplied by
+    1 + z + z^2/2 + z^3/6 (coefficients 1/3, 1/2, 1/6)."""
+
+    class _Field:
+        def poisson(self, *a, **k):
+            pass
+    case = oracle.Case(n=3, nx=2, nu=2, nv=2)
+    f0 = case.dist.arr.copy()
+    dt = 0.1
+    st = oracle.Stepper(dt)
+    st.ssp_rk_update(case.dist, case.basis, _Field(), case.grids, _LinearFlux(z / dt))
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    assert np.allclose(interior(case.dist.arr), R * interior(f0), rtol=1e-14, atol=0)
+    assert oracle.SSP_RK_LINEAR[3] == [1 / 3, 1 / 2, 1 / 6]

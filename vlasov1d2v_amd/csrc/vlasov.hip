@@ -19,6 +19,7 @@
 #include "vl_stage_rs.h"
 #include "vl_ensemble.h"
 #include "vl_init.h"
+#include "vl_central.h"
 
 #include <vector>
 
@@ -45,6 +46,10 @@ struct vl_geom {
     float *d_bg32;         // reduced forms: scaled operator images of f0 (row-split kernel)
     unsigned *d_mxfrag;    // row-split kernel: operator fragments per lane (BF16, FP16)
     OpTables *d_tab;       // device copy of tab (read by the stage kernel)
+    // central-flux Poisson (optional): inverse operator (K+1)^2, gradient
+    // operator transposed K*K, mass n*n, and its scratch
+    double *d_cf_inv, *d_cf_gradT, *d_cf_mass, *d_cf_rhs, *d_cf_phi, *d_cf_e;
+    double2 *d_cf_coef;
     std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
     double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
     void **stage_events;   // optional: 6 hipEvent_t recorded around the stage kernels
@@ -706,8 +711,9 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
 
 int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
-    void *bufs[] = {h->d_xnodes, h->d_unodes, h->d_vnodes, h->d_wts,  h->d_kwave, h->d_T,
-                    h->d_phases, h->d_bg,     h->d_bg32,   h->d_tab, h->d_mxfrag};
+    void *bufs[] = {h->d_xnodes, h->d_unodes, h->d_vnodes,  h->d_wts,     h->d_kwave,  h->d_T,
+                    h->d_phases, h->d_bg,     h->d_bg32,    h->d_tab,     h->d_mxfrag, h->d_cf_inv,
+                    h->d_cf_gradT, h->d_cf_mass, h->d_cf_rhs, h->d_cf_phi, h->d_cf_e,  h->d_cf_coef};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -1045,6 +1051,70 @@ int vl_resident_layout(vl_geom_t h, int *layout) {
 int vl_set_stage_events(vl_geom_t h, void **events) {
     if (int r = check_handle(h)) return r;
     h->stage_events = events;
+    return VL_OK;
+}
+
+int vl_set_central_flux(vl_geom_t h, const double *inv_op, const double *grad_op, const double *mass) {
+    if (int r = check_handle(h)) return r;
+    if (!inv_op || !grad_op || !mass) return fail(VL_E_NULL, "vl_set_central_flux: null table");
+    const int K = h->nx * h->n, n = h->n;
+    const size_t ni = (size_t)(K + 1) * (K + 1), ng = (size_t)K * K;
+    for (size_t i = 0; i < ni; ++i)
+        if (!std::isfinite(inv_op[i])) return fail(VL_E_NOTFINITE, "vl_set_central_flux: non-finite inverse");
+    std::vector<double> gT(ng);
+    for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c) gT[(size_t)c * K + r] = grad_op[(size_t)r * K + c];
+    hipError_t e = hipSuccess;
+    if (!h->d_cf_inv) {
+        e = hipMalloc((void **)&h->d_cf_inv, ni * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_gradT, ng * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_mass, (size_t)n * n * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_rhs, (size_t)(K + 1) * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_phi, (size_t)K * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_e, (size_t)K * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->d_cf_coef, (size_t)h->M * sizeof(double2));
+    }
+    if (e == hipSuccess) e = hipMemcpy(h->d_cf_inv, inv_op, ni * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_cf_gradT, gT.data(), ng * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_cf_mass, mass, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "vl_set_central_flux");
+    return VL_OK;
+}
+
+int vl_poisson_central(vl_geom_t h, const double *cd, double *E, double *phi, int anti_alias, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!cd || !E) return fail(VL_E_NULL, "vl_poisson_central: null pointer");
+    if (!aligned(cd, 8) || !aligned(E, 8) || (phi && !aligned(phi, 8)))
+        return fail(VL_E_ALIGN, "vl_poisson_central: misaligned pointer");
+    if (!h->d_cf_inv) return fail(VL_E_ARG, "vl_poisson_central: call vl_set_central_flux first");
+    hipStream_t st = (hipStream_t)stream;
+    const int K = h->nx * h->n, n = h->n, M = h->M;
+    const double J = h->jx;
+    const int gd = (M + kDftModes - 1) / kDftModes, gk = (K + kBlock - 1) / kBlock;
+    double *ph = phi ? phi : h->d_cf_phi;
+    // rhs, then phi = (inv_op rhs)[:-1] / J^2   (main/elliptic.py:93-98)
+    hipLaunchKernelGGL(k_cf_rhs, dim3((K + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cd, h->d_cf_mass,
+                       h->d_cf_rhs, h->nx, n, h->pcoef);
+    hipLaunchKernelGGL(k_matvec, dim3(K), dim3(kBlock), 0, st, h->d_cf_inv, h->d_cf_rhs,
+                       anti_alias ? h->d_cf_e : ph, K, K + 1, 1.0 / (J * J), 0);
+    if (anti_alias) {                                                 // main/elliptic.py:101-103
+        hipLaunchKernelGGL(k_dft_plain, dim3(gd), dim3(kBlock), 0, st, h->d_cf_e, h->d_T, h->d_cf_coef, K, M,
+                           h->dx, h->length);
+        hipLaunchKernelGGL(k_dft_plain_sum, dim3(gk), dim3(kBlock), 0, st, h->d_cf_coef, h->d_phases, ph, K, M, 0,
+                           0);
+    }
+    // E = -(J grad_op . phi)   (main/elliptic.py:106-108)
+    hipLaunchKernelGGL(k_matvec, dim3(K), dim3(kBlock), 0, st, h->d_cf_gradT, ph, anti_alias ? h->d_cf_e : E, K,
+                       K, -J, anti_alias ? 0 : n);
+    if (anti_alias) {                                                 // main/elliptic.py:111-113
+        hipLaunchKernelGGL(k_dft_plain, dim3(gd), dim3(kBlock), 0, st, h->d_cf_e, h->d_T, h->d_cf_coef, K, M,
+                           h->dx, h->length);
+        hipLaunchKernelGGL(k_dft_plain_sum, dim3(gk), dim3(kBlock), 0, st, h->d_cf_coef, h->d_phases, E, K, M, n,
+                           n);
+    } else {
+        hipLaunchKernelGGL(k_field_ghosts, dim3(1), dim3(64), 0, st, E, h->nx, n);
+    }
+    VL_HIP(hipGetLastError());
     return VL_OK;
 }
 

@@ -26,11 +26,28 @@ NONLINEAR_SSP_RK = {
         [1 / 3, 2 / 3, 2 / 3]],
 }
 
+# Linear SSP-RK coefficients by order (main/timestep.py:11-21)
+SSP_RK_LINEAR = {
+    1: [1],
+    2: [1 / 2, 1 / 2],
+    3: [1 / 3, 1 / 2, 1 / 6],
+    4: [3 / 8, 1 / 3, 1 / 4, 1 / 24],
+    5: [11 / 30, 3 / 8, 1 / 6, 1 / 12, 1 / 120],
+    6: [53 / 144, 11 / 30, 3 / 16, 1 / 18, 1 / 48, 1 / 720],
+    7: [103 / 280, 53 / 144, 11 / 60, 3 / 48, 1 / 72, 1 / 240, 1 / 5040],
+    8: [2119 / 5760, 103 / 280, 53 / 288, 11 / 180, 1 / 64, 1 / 360, 1 / 1440, 1 / 40320],
+}
+
 # Cockburn-Shu Courant numbers [time_order][space_order - 1] (main/timestep.py:24-33)
 COURANT_NUMBERS = {
     1: [1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0],
     2: [1.0, 0.333],
     3: [1.256, 0.409, 0.209, 0.130, 0.089, 0.066, 0.051, 0.040, 0.033],
+    4: [1.392, 0.464, 0.235, 0.145, 0.100, 0.073, 0.056, 0.045, 0.037],
+    5: [1.608, 0.534, 0.271, 0.167, 0.115, 0.085, 0.065, 0.052, 0.042],
+    6: [1.776, 0.592, 0.300, 0.185, 0.127, 0.093, 0.072, 0.057, 0.047],
+    7: [1.977, 0.659, 0.333, 0.206, 0.142, 0.104, 0.080, 0.064, 0.052],
+    8: [2.156, 0.718, 0.364, 0.225, 0.154, 0.114, 0.087, 0.070, 0.057],
 }
 
 
@@ -45,13 +62,16 @@ def get_max_speeds(grids, elliptic, acc):
 
 class Stepper:
     def __init__(self, time_order, space_order, write_time, final_time, linear=False, dt=None):
-        if linear:
-            raise NotImplementedError("the linear SSP-RK update (main/timestep.py:189-229) is not on this path")
-        if time_order != 3:
+        self.linear = bool(linear)
+        if self.linear:
+            if time_order not in SSP_RK_LINEAR:
+                raise ValueError(f"no linear SSP-RK coefficients of order {time_order}")
+        elif time_order != 3:
             raise NotImplementedError("the fused GPU step implements the SSP-RK3 row only")
         self.time_order = int(time_order)
         self.space_order = int(space_order)
-        self.coefficients = np.array(NONLINEAR_SSP_RK[self.time_order])
+        self.coefficients = np.array(SSP_RK_LINEAR[self.time_order] if self.linear
+                                     else NONLINEAR_SSP_RK[self.time_order])
         # The reference's table stops at 9 nodes (IndexError there for 10);
         # here a missing entry only matters for the adaptive (CFL) time step.
         row = COURANT_NUMBERS[self.time_order]
@@ -176,6 +196,38 @@ class Stepper:
                 _lib.stream_ptr()), "vl_rk3_step")
         self._rho_token = (arr.data_ptr(), arr._version, ws.data_ptr())
 
+    def ssp_rk_update(self, distribution, basis, grids, elliptic, dg_flux, refs=None):
+        """The linear SSP-RK update (main/timestep.py:189-229) on the GPU, in
+        the reference's order, with the central-flux Poisson solve
+        (``elliptic.poisson``; build_central_flux_operator and invert first).
+        FP64 state in the reference layout."""
+        arr = distribution.arr
+        _check_state(arr, grids.shape, "ssp_rk_update")
+        if self.dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+        c = self.coefficients
+        ng = grids.no_ghost_slice
+        geo = distribution._geom()
+        lib = _lib.load()
+
+        def field_of(copy):
+            _lib.check(lib.vl_swap_ghosts(geo.handle, _lib.ptr(copy), _lib.stream_ptr()), "vl_swap_ghosts")
+            density = distribution.moment_zero_of_arr(copy)
+            elliptic.poisson(density.mean() - density, grids.x, basis.b1)
+
+        distribution.swap_ghost_cells_of_array()
+        copy = arr.clone()                                  # copy_for_explicit_advance (main/grid.py:198-199)
+        arr[ng] = c[0] * arr[ng]
+        for i in range(1, self.time_order):
+            field_of(copy)
+            copy[ng] += self.dt * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
+            arr[ng] += c[i] * copy[ng]
+        field_of(copy)
+        arr[ng] += (c[-1] * self.dt) * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
+        self._rho_token = None
+        if self.fixed_dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+
     def nonlinear_ssp_rk_unfused(self, distribution, basis, elliptic, grids, dg_flux, refs=None):
         """The same step composed from the per-operator GPU entry points in the
         reference's order (main/timestep.py:141-187)."""
@@ -218,7 +270,9 @@ class Stepper:
         t0 = _time.time()
         dev = distribution.arr.device
         e0 = elliptic.electric_energy(grids.x)
-        if self.fixed_dt is None and device_dt and self.courant is not None \
+        if self.linear:
+            self._main_loop_linear(distribution, basis, elliptic, grids, dg_flux, refs, steps, e0)
+        elif self.fixed_dt is None and device_dt and self.courant is not None \
                 and self._acc(dg_flux, refs) == dg_flux.acceleration_coefficient:
             self._main_loop_device_dt(distribution, basis, elliptic, grids, dg_flux, refs, steps, graph_steps, e0)
         elif self.fixed_dt is None:
@@ -263,6 +317,20 @@ class Stepper:
             torch.cuda.synchronize(dev)
             self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
         self.elapsed = _time.time() - t0
+
+    def _main_loop_linear(self, distribution, basis, elliptic, grids, dg_flux, refs, steps, e0):
+        """main/timestep.py:81-139 with the linear update of :189-229."""
+        if self.fixed_dt is None:
+            self._adapt(grids, elliptic, dg_flux, refs)
+        self.field_energy = np.append(self.field_energy, float(e0.item()))
+        done = 0
+        while (self.time < self.final_time) if steps is None else (done < steps):
+            done += 1
+            self.ssp_rk_update(distribution, basis, grids, elliptic, dg_flux, refs)
+            self.time += self.dt
+            self.steps_counter += 1
+            self.time_array = np.append(self.time_array, self.time)
+            self.field_energy = np.append(self.field_energy, float(elliptic.electric_energy(grids.x).item()))
 
     def _main_loop_device_dt(self, distribution, basis, elliptic, grids, dg_flux, refs, steps, graph_steps, e0):
         """main/timestep.py:81-139 with the CFL rule on the device: the clock
