@@ -49,6 +49,9 @@
 // element, one slot per tile element.
 // The global loads of the next tile are in flight while the current one is
 // computed; the epilogue works on whole rows (16-byte loads and stores).
+// Stages 2/3 read f^n of the current element with direct-to-LDS buffer
+// loads issued at the start of the tile (no registers held across the
+// passes); the epilogue waits for them and reads the rows from LDS.
 #pragma once
 #include "vl_stage_mx.h"
 
@@ -175,7 +178,12 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     __shared__ __attribute__((aligned(16))) float xa[TV * XE];
     __shared__ float red[TV * ROWS];   // per-row moment sums, wave-private rows
     // per-lane sink of the 16-byte stores of lanes that hold no output node
-    __shared__ __attribute__((aligned(16))) float sink[TV * 64 * 4];
+    // (shared by the waves: never read)
+    __shared__ __attribute__((aligned(16))) float sink[64 * 4];
+    // f^n of this wave's element, staged by direct-to-LDS loads: rows (16 B
+    // each) then the tail dwords
+    constexpr int FNR = ROWS * 4, FNW = ((ROWS * 4 + (N == 9 ? 64 : 0)) + 3) & ~3;
+    __shared__ __attribute__((aligned(16))) unsigned fnst[MODE == kModeStage ? TV * FNW : 4];
 
     const int tid = threadIdx.x;
     const int ln = tid & 63;
@@ -360,6 +368,22 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     for (int ex = pc.ex_begin; ex <= pc.ex_end; ++ex) {
         const bool has_next = ex < pc.ex_end;
         fill(cur);
+        if (MODE == kModeStage && active) {
+            // f^n rows of this element straight into LDS: lane l -> row l,
+            // then rows 64.. (lanes < ROWS - 64); n = 9: the tail dwords
+            const int bfn = (int)(ebase(ex, eu, ev) * 2);
+            unsigned *st0 = fnst + el * FNW;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)st0, 16,
+                                                     bfn + ln * 16, 0, 0, 0);
+            if (ln < ROWS - 64)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + 256),
+                                                         16, bfn + (64 + ln) * 16, 0, 0, 0);
+            if (N == 9)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + FNR),
+                                                         4, ln < (ROWS + 1) / 2 ? bfn + 16 * ROWS + 4 * ln : kOob,
+                                                         0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         if (has_next) load(ex + 1, nxt);
         const int64_t b = ebase(ex, eu, ev);
         wave_sync_lds();
@@ -371,7 +395,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             const int rdx = min(lj, 11) * PSD + 2 * min(lg, 2);    // x-slot lj, v-slots 4 lg ..
             const int rduv = min(lj, 11) * RSD + 2 * min(lg, 2);   // u-slot lj, v-slots 4 lg ..
             const bool wr = lj < N && lg <= GMAX;
-            float *const snk = sink + (el * 64 + ln) * 4;
+            float *const snk = sink + ln * 4;
             const unsigned selp = lg == 2 ? kSelPos : kSelKeep, seln = lg == 2 ? kSelNeg : kSelKeep;
             // ---- x: X[ix][iu][v-slot] = A_x g (the x speed is applied in the u/v pass)
 #pragma unroll
@@ -422,6 +446,9 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             wave_sync_lds();
             // ---- epilogue: whole rows ----------------------------------------
             const int64_t bo = ebase(ex, eu, ev);
+            if (MODE == kModeStage) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the staged f^n is in LDS
+            }
 #pragma unroll 1
             for (int k = 0; k < NRI; ++k) {
                 const int r = ln + 64 * k;
@@ -429,8 +456,9 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                     vl_u32x4 fnr = {0u, 0u, 0u, 0u};
                     unsigned fnt = 0u;
                     if (MODE == kModeStage) {
-                        fnr = buf_ld16(rfn, (int)(b * 2) + r * 16);
-                        fnt = N == 9 ? ld32(rfn, ((int)(b * 2) + (8 * ROWS + r) * 2) & ~3) : 0u;
+                        const unsigned *st0 = fnst + el * FNW;
+                        fnr = *reinterpret_cast<const vl_u32x4 *>(st0 + r * 4);
+                        fnt = N == 9 ? st0[FNR + (r >> 1)] : 0u;
                     }
                     const int iu = r - (r / N) * N;
                     const int ix = r / N;
