@@ -1,6 +1,7 @@
 """Error report of the reduced-precision storage forms against the FP64 form.
 
     python -m vlasov1d2v_amd.errors [--workload ring64] [--t-end 20] [--forms f32,bf16,f16]
+                                    [--ic eigen --k0 0.0886 --om -0.349j --om-pc 10 --delta-n 1e-3]
 
 Every form runs the same seeded workload to ``t_end`` with the same fixed dt
 (steps replayed from a HIP graph).  The FP64 GPU form is the reference: it is
@@ -30,10 +31,14 @@ import time
 import numpy as np
 
 
-def run_form(w, storage, t_end, dt, graph_steps, device):
+def run_form(w, storage, t_end, dt, graph_steps, device, mfma=True, **ic):
     import torch
+    from . import _lib
+    from .grid import STORAGE_CODES
     from .workload import Run
-    run = Run.from_workload(w, dt=dt, device=device, storage=storage)
+    run = Run.from_workload(w, dt=dt, device=device, storage=storage, **ic)
+    if not mfma:
+        run.grids.geometry(STORAGE_CODES[storage]).set_tuning(_lib.VL_TUNE_MFMA, 0)
     steps = int(round(t_end / dt))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -75,23 +80,37 @@ def main(argv=None):
     ap.add_argument("--fit", default="", help="fit window t0,t1 (default: the second half of the run)")
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--save", default="", help="write the time and energy series of every form to this .npz")
+    ap.add_argument("--no-mfma", action="store_true",
+                    help="16-bit forms on the vector-ALU stage kernel (element-major) instead of the matrix cores")
+    ap.add_argument("--ic", default="modulated", choices=["modulated", "eigen"],
+                    help="the workload's modulated ring, or the reference driver's ring + Bernstein eigenfunction")
+    ap.add_argument("--k0", type=float, default=0.0, help="wave number (default: the workload's)")
+    ap.add_argument("--om", type=complex, default=-0.349j, help="eigenfunction frequency (ic=eigen)")
+    ap.add_argument("--om-pc", type=float, default=10.0, help="omega_p / omega_c of the eigenfunction (ic=eigen)")
+    ap.add_argument("--delta-n", type=float, default=1e-3, help="eigenfunction amplitude (ic=eigen)")
     args = ap.parse_args(argv)
     w = CONFIGS[args.workload]
+    if args.k0 > 0:
+        import dataclasses
+        w = dataclasses.replace(w, k0=args.k0)
+    ic = {} if args.ic == "modulated" else dict(ic="eigen", om=args.om, om_pc=args.om_pc, delta_n=args.delta_n)
     dt = args.dt if args.dt > 0 else w.dt
     window = tuple(float(x) for x in args.fit.split(",")) if args.fit else (0.5 * args.t_end, args.t_end)
     dev = torch.device("cuda:0")
-    t, e64, f64, s64, _ = run_form(w, "f64", args.t_end, dt, args.graph_steps, dev)
+    t, e64, f64, s64, _ = run_form(w, "f64", args.t_end, dt, args.graph_steps, dev, **ic)
     g64 = growth_rate(t, e64, window)
     ge64 = envelope_rate(t, e64, e64)
     nf64 = torch.linalg.vector_norm(f64).item()
-    out = {"workload": w.name, "dof": w.dof, "dt": dt, "t_end": args.t_end, "steps": len(t) - 1,
+    out = {"workload": w.name, "dof": w.dof, "k0": w.k0, "ic": args.ic if not ic else
+           {"ic": "eigen", "om": str(args.om), "om_pc": args.om_pc, "delta_n": args.delta_n},
+           "dt": dt, "t_end": args.t_end, "steps": len(t) - 1,
            "fit_window": list(window),
            "f64": {"seconds": s64, "gamma": g64, "gamma_env": ge64, "energy_end": float(e64[-1]),
                    "energy_range": [float(e64.min()), float(e64.max())]},
            "forms": {}}
     series = {"t": t, "f64": e64}
     for storage in [s for s in args.forms.split(",") if s]:
-        _, e, f, sec, _ = run_form(w, storage, args.t_end, dt, args.graph_steps, dev)
+        _, e, f, sec, _ = run_form(w, storage, args.t_end, dt, args.graph_steps, dev, mfma=not args.no_mfma, **ic)
         series[storage] = e
         g = growth_rate(t, e, window)
         ge = envelope_rate(t, e, e64)
