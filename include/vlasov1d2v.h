@@ -235,6 +235,17 @@ int vl_set_stage_events(vl_geom_t h, void **events);
  * (nu+2)*n*(nv+2)*n values.  Only used when storage != VL_F64. */
 int vl_set_background(vl_geom_t h, const double *f0_host);
 
+/* Run-file records.  vl_export_f32 writes the state f (the handle's storage
+ * form, reference layout, ghosts included) as float32 values of f itself
+ * (f0 + g / scale for the reduced forms) into `out` (same shape): the "pdf"
+ * record that RunData.save_data stores (main/data_management.py:37-52,
+ * `f['pdf'][-1] = distribution`, dtype 'f').  vl_import_f32 is the inverse
+ * for resuming from a record: f = in (FP64), or g = (in - f0) * scale rounded
+ * to nearest (reduced forms).  The reduced forms need vl_set_background
+ * first.  No counterpart for the import: the reference has no resume. */
+int vl_export_f32(vl_geom_t h, const void *f, float *out, void *stream);
+int vl_import_f32(vl_geom_t h, const float *in, void *f, void *stream);
+
 /* ---- Ensembles (no counterpart in the reference, which steps one run per
  * process through main/main.py:165 -> Stepper.main_loop).  Many independent
  * runs -- each its own geometry (n, grid, physics), state and workspace --

@@ -24,7 +24,7 @@ EXPORTS = (
     "vl_ensemble_create", "vl_ensemble_destroy", "vl_ensemble_bind", "vl_ensemble_set_schedule",
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
-    "vl_poisson_central",
+    "vl_poisson_central", "vl_export_f32", "vl_import_f32",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -99,6 +99,8 @@ def load(path=LIB_PATH):
         "vl_init_eigen": [vp, ctypes.POINTER(VlEigenDesc), vp, vp],
         "vl_set_central_flux": [vp, vp, vp, vp],
         "vl_poisson_central": [vp, vp, vp, vp, i32, vp],
+        "vl_export_f32": [vp, vp, vp, vp],
+        "vl_import_f32": [vp, vp, vp, vp],
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],

@@ -36,6 +36,50 @@ def compile_cmd(out=LIB_PATH, extra=()):
             "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
 
 
+IO_LIB_NAME = "libvlasov1d2v_io.so"
+IO_LIB_PATH = os.path.join(HERE, IO_LIB_NAME)
+IO_SOURCES = ["vl_io.c"]
+
+
+def hdf5_root():
+    """Prefix of the HDF5 1.10+ C library (``include/hdf5.h``, ``lib/libhdf5.so``):
+    ``$HDF5_ROOT``, else the image's /opt/conda, else /usr; None if absent."""
+    for cand in (os.environ.get("HDF5_ROOT"), "/opt/conda", "/usr"):
+        if cand and os.path.exists(os.path.join(cand, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(cand, "lib", "libhdf5.so")):
+            return cand
+    return None
+
+
+def io_compile_cmd(root, out=IO_LIB_PATH):
+    lib = os.path.join(root, "lib")
+    return ["gcc", "-O2", "-std=gnu99", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+            "-I", os.path.join(root, "include"), "-o", out] + \
+        [os.path.join(CSRC, s) for s in IO_SOURCES] + \
+        ["-L", lib, "-Wl,-rpath," + lib, "-lhdf5", "-lpthread"]
+
+
+def build_io(verbose=True):
+    """Build the run-file library (HDF5) next to the HIP library.  Returns
+    its path, or None when no HDF5 C library is installed (the stepping
+    library does not need it)."""
+    root = hdf5_root()
+    if root is None:
+        if verbose:
+            print("HDF5 C library not found: libvlasov1d2v_io.so not built", flush=True)
+        return None
+    tmp = IO_LIB_PATH + ".tmp"
+    cmd = io_compile_cmd(root, tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"gcc failed with exit code {r.returncode}")
+    os.replace(tmp, IO_LIB_PATH)
+    return IO_LIB_PATH
+
+
 def build(verbose=True):
     tmp = LIB_PATH + ".tmp"
     cmd = compile_cmd(tmp)
@@ -51,3 +95,4 @@ def build(verbose=True):
 
 if __name__ == "__main__":
     build()
+    build_io()

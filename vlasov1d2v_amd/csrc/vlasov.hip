@@ -20,6 +20,7 @@
 #include "vl_ensemble.h"
 #include "vl_init.h"
 #include "vl_central.h"
+#include "vl_export.h"
 
 #include <vector>
 
@@ -51,6 +52,7 @@ struct vl_geom {
     double *d_cf_inv, *d_cf_gradT, *d_cf_mass, *d_cf_rhs, *d_cf_phi, *d_cf_e;
     double2 *d_cf_coef;
     std::vector<double> f0_host;   // reduced forms: background f0 (u, v), host copy
+    double *d_f0;          // reduced forms: device copy of f0 (run-file records)
     double scale, inv_scale;       // reduced forms: stored g = (f - f0) * scale
     void **stage_events;   // optional: 6 hipEvent_t recorded around the stage kernels
     // adaptive time step (main/timestep.py:231-242): Courant number, cell
@@ -713,7 +715,8 @@ int vl_geom_destroy(vl_geom_t h) {
     if (!h) return VL_OK;
     void *bufs[] = {h->d_xnodes, h->d_unodes, h->d_vnodes,  h->d_wts,     h->d_kwave,  h->d_T,
                     h->d_phases, h->d_bg,     h->d_bg32,    h->d_tab,     h->d_mxfrag, h->d_cf_inv,
-                    h->d_cf_gradT, h->d_cf_mass, h->d_cf_rhs, h->d_cf_phi, h->d_cf_e,  h->d_cf_coef};
+                    h->d_cf_gradT, h->d_cf_mass, h->d_cf_rhs, h->d_cf_phi, h->d_cf_e,  h->d_cf_coef,
+                    h->d_f0};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete h;
@@ -1166,6 +1169,72 @@ int vl_set_background(vl_geom_t h, const double *f0_host) {
         if (!std::isfinite(f0_host[i])) return fail(VL_E_NOTFINITE, "vl_set_background: non-finite f0");
     h->f0_host.assign(f0_host, f0_host + cnt);
     if (int r = upload_background(h)) return r;
+    if (!h->d_f0) VL_HIP(hipMalloc((void **)&h->d_f0, cnt * sizeof(double)));
+    VL_HIP(hipMemcpy(h->d_f0, f0_host, cnt * sizeof(double), hipMemcpyHostToDevice));
+    return VL_OK;
+}
+
+// Run-file records (RunData.save_data, main/data_management.py:37-52): the
+// state as float32 f in the reference layout, and back.
+static int record_check(vl_geom_t h, const void *a, const void *b, const char *what) {
+    if (int r = check_handle(h)) return r;
+    if (!a || !b) return fail(VL_E_NULL, "%s: null pointer", what);
+    if (h->storage != VL_F64 && !h->d_f0) return fail(VL_E_ARG, "%s: reduced storage needs vl_set_background", what);
+    return VL_OK;
+}
+
+int vl_export_f32(vl_geom_t h, const void *f, float *out, void *stream) {
+    if (int r = record_check(h, f, out, "vl_export_f32")) return r;
+    const size_t eb = storage_bytes(h->storage);
+    if (!aligned(f, eb) || !aligned(out, 4)) return fail(VL_E_ALIGN, "vl_export_f32: misaligned");
+    const int64_t plane = (int64_t)(h->nu + 2) * h->n * (h->nv + 2) * h->n;
+    const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock), (unsigned)((h->nx + 2) * h->n));
+    hipStream_t st = (hipStream_t)stream;
+    switch (h->storage) {
+    case VL_F64:
+        hipLaunchKernelGGL(k_export_f32<0>, grid, dim3(kBlock), 0, st, (const double *)f, h->d_f0, out, plane, 1.0);
+        break;
+    case VL_F32:
+        hipLaunchKernelGGL(k_export_f32<1>, grid, dim3(kBlock), 0, st, (const float *)f, h->d_f0, out, plane,
+                           h->inv_scale);
+        break;
+    case VL_BF16:
+        hipLaunchKernelGGL(k_export_f32<2>, grid, dim3(kBlock), 0, st, (const uint16_t *)f, h->d_f0, out, plane,
+                           h->inv_scale);
+        break;
+    default:
+        hipLaunchKernelGGL(k_export_f32<3>, grid, dim3(kBlock), 0, st, (const uint16_t *)f, h->d_f0, out, plane,
+                           h->inv_scale);
+        break;
+    }
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_import_f32(vl_geom_t h, const float *in, void *f, void *stream) {
+    if (int r = record_check(h, in, f, "vl_import_f32")) return r;
+    const size_t eb = storage_bytes(h->storage);
+    if (!aligned(f, eb) || !aligned(in, 4)) return fail(VL_E_ALIGN, "vl_import_f32: misaligned");
+    const int64_t plane = (int64_t)(h->nu + 2) * h->n * (h->nv + 2) * h->n;
+    const dim3 grid((unsigned)((plane + kBlock - 1) / kBlock), (unsigned)((h->nx + 2) * h->n));
+    hipStream_t st = (hipStream_t)stream;
+    switch (h->storage) {
+    case VL_F64:
+        hipLaunchKernelGGL(k_import_f32<0>, grid, dim3(kBlock), 0, st, in, h->d_f0, (double *)f, plane, 1.0);
+        break;
+    case VL_F32:
+        hipLaunchKernelGGL(k_import_f32<1>, grid, dim3(kBlock), 0, st, in, h->d_f0, (float *)f, plane, h->scale);
+        break;
+    case VL_BF16:
+        hipLaunchKernelGGL(k_import_f32<2>, grid, dim3(kBlock), 0, st, in, h->d_f0, (uint16_t *)f, plane,
+                           h->scale);
+        break;
+    default:
+        hipLaunchKernelGGL(k_import_f32<3>, grid, dim3(kBlock), 0, st, in, h->d_f0, (uint16_t *)f, plane,
+                           h->scale);
+        break;
+    }
+    VL_HIP(hipGetLastError());
     return VL_OK;
 }
 
