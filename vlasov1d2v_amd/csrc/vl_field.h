@@ -235,8 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_dft_forward(const double *__restrict
 
 // Inverse sums at the nodes: E[1:-1] = Re sum_m Ehat_m phases[m][k] (and phi),
 // periodic ghosts, and -- when energy/maxabs are requested -- the field energy
-// 0.5 sum w E^2 and max|acc E|, reduced in a fixed order by the last workgroup
-// to finish (ticket counter in the workspace; it returns to zero).
+// 0.5 sum w E^2 and max|acc E|, reduced in a fixed order by all threads of the
+// last workgroup to finish (ticket counter in the workspace; it returns to zero).
 // LDS of the inverse sums besides the 2*M coefficients
 struct InvLds {
     double pe[kBlock], pp[kBlock];
@@ -335,20 +335,24 @@ __device__ __forceinline__ void dft_inverse_body(
     }
     __syncthreads();
     if (!is_last) return;
+    // the last workgroup: thread t sums partials t, t + kBlock, ... (double-double),
+    // then a fixed tree over the threads (pe / pp are free again)
+    __threadfence();
+    DD s;
+    double mx = 0.0;
+    const volatile double *vp = reinterpret_cast<const volatile double *>(en_part);
+    const volatile double *vm = mx_part;
+    for (int b = tid; b < nblocks; b += kBlock) {
+        DD d;
+        d.hi = vp[2 * b];
+        d.lo = vp[2 * b + 1];
+        s.add(d);
+        mx = fmax(mx, vm[b]);
+    }
+    const double tot = block_sum_dd(s, pe, pp);
+    mx = block_max(mx, pe);
     if (tid == 0) {
-        __threadfence();
-        DD s;
-        double mx = 0.0;
-        const volatile double *vp = reinterpret_cast<const volatile double *>(en_part);
-        const volatile double *vm = mx_part;
-        for (int b = 0; b < nblocks; ++b) {
-            DD d;
-            d.hi = vp[2 * b];
-            d.lo = vp[2 * b + 1];
-            s.add(d);
-            mx = fmax(mx, vm[b]);
-        }
-        if (energy) energy[0] = 0.5 * s.value();
+        if (energy) energy[0] = 0.5 * tot;
         if (maxabs) maxabs[0] = mx;
         if (a.dt_next) a.dt_next[0] = cfl_dt(a, mx);
         atomicExch(ticket, 0u);
