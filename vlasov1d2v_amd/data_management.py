@@ -97,8 +97,7 @@ class _Staging:
         import torch
         self.dev = torch.empty(shape, dtype=torch.float32, device=device)
         self.host = [torch.empty(shape, dtype=torch.float32, pin_memory=True) for _ in range(2)]
-        self.events = [None, None]
-        self.pending = [None, None]
+        self.pending = [None, None]      # write future of each host buffer
         self.next = 0
 
 
@@ -116,6 +115,7 @@ class RunData:
             refs.info_file(self.info_name, geometry, time)
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         self._staging = None
+        self._last = None
 
     # -- state conversion ------------------------------------------------
     def _pdf_record(self, distribution):
@@ -199,7 +199,7 @@ class RunData:
     def flush(self):
         """Waits for every pending write (re-raises a failed one)."""
         st = self._staging
-        futs = [getattr(self, "_last", None)] + (list(st.pending) if st is not None else [])
+        futs = [self._last] + (list(st.pending) if st is not None else [])
         for f in futs:
             if f is not None:
                 f.result()
