@@ -130,6 +130,7 @@ __device__ __forceinline__ void ens_stage_pencil(const EnsRun &r, int p, int sta
     const OpTables *tabp = r.tab;
     const Pencil pc = decode_pencil(s, p);
     C *ft = reinterpret_cast<C *>(lds + LY::o_ft);
+    C *vh = reinterpret_cast<C *>(lds + LY::o_vh);
     C *xl = reinterpret_cast<C *>(lds + LY::o_xl);
     C *xr = reinterpret_cast<C *>(lds + LY::o_xr);
     C *uh = reinterpret_cast<C *>(lds + LY::o_uh);

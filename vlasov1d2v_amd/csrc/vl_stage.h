@@ -142,26 +142,40 @@ __device__ __forceinline__ Pencil decode_pencil(const StageArgs &s, int p) {
 
 // LDS arrays of one stage workgroup.  The single-run kernel declares them as
 // separate __shared__ arrays; the ensemble kernel carves them out of one
-// buffer (byte offsets below, FP64 arrays first).
+// buffer (byte offsets below, the 16-byte aligned arrays first).
+//   ft, acc  tile values and RHS accumulator, element-major like the state's
+//            element-major layout: node (el, ix, iu, iv) at el EP + (ix N + iu) N + iv,
+//            so that the 16-byte rows of that layout move as whole vectors
+//   vh       v-halo nodes of the tile rows r = ix N + iu: [side][r]
+//   xl, xr   left / right x-halo planes [el][iu][iv]
+//   uh       u-halo faces [side][el][ix][iv]
+// With one line per thread in the order (el, a, b) the x-line reads of ft and
+// the v-line reads (9 consecutive values, lanes 9 apart) are free of LDS bank
+// conflicts on 64 x 4-byte banks, and the epilogue's 16-byte accesses are
+// contiguous across lanes.
 template <int N, int S>
 struct StageLds {
     using C = typename Store<S>::C;
     static constexpr bool BG = Store<S>::kDelta;
-    static constexpr int P = TileShape<N>::P, PP = TileShape<N>::PP;
-    static constexpr int n_ft = N * N * PP;   // tile rows (ix,iu) x v-nodes with halo
-    static constexpr int n_x = N * P;         // left / right x halo [iu][q]
-    static constexpr int n_uh = 2 * N * P;    // u halo [side][ix][q]
-    static constexpr int n_acc = N * N * P;   // output accumulator rows (ix,iu) x q
-    static constexpr int n_bgs = BG ? 3 * N * P : 1;   // background slab [table][iu][q]
-    static constexpr int n_red = BG ? N * P : 1;       // moment scratch (the FP64 form reuses uh)
-    static constexpr size_t o_bgs = 0;
+    static constexpr int P = TileShape<N>::P;
+    static constexpr int EP = elem_pitch(N);
+    static constexpr int n_ft = TileShape<N>::TV * EP;   // tile values, element-major
+    static constexpr int n_acc = n_ft;                   // RHS accumulator, same layout
+    static constexpr int n_vh = 2 * N * N;               // v halo [side][ix*N + iu]
+    static constexpr int n_x = N * P;                    // left / right x halo [el][iu][iv]
+    static constexpr int n_uh = 2 * N * P;               // u halo [side][el][ix][iv]
+    static constexpr int n_bgs = BG ? 3 * N * P : 1;     // background slab [table][iu][q]
+    static constexpr int n_red = BG ? N * P : 1;         // moment scratch (the FP64 form reuses uh)
+    static constexpr size_t o_ft = 0;
+    static constexpr size_t o_acc = o_ft + (size_t)n_ft * sizeof(C);
+    static constexpr size_t o_bgs = o_acc + (size_t)n_acc * sizeof(C);
     static constexpr size_t o_red = o_bgs + (size_t)n_bgs * 8;
-    static constexpr size_t o_ft = o_red + (size_t)n_red * 8;
-    static constexpr size_t o_xl = o_ft + (size_t)n_ft * sizeof(C);
+    static constexpr size_t o_vh = o_red + (size_t)n_red * 8;
+    static constexpr size_t o_xl = o_vh + (size_t)n_vh * sizeof(C);
     static constexpr size_t o_xr = o_xl + (size_t)n_x * sizeof(C);
     static constexpr size_t o_uh = o_xr + (size_t)n_x * sizeof(C);
-    static constexpr size_t o_acc = o_uh + (size_t)n_uh * sizeof(C);
-    static constexpr size_t bytes = o_acc + (size_t)n_acc * sizeof(C);
+    static constexpr size_t bytes = o_uh + (size_t)n_uh * sizeof(C);
+    static_assert((EP * sizeof(C)) % 16 == 0, "ft / acc elements start 16-byte aligned");
     static_assert(BG || (o_uh % 8 == 0), "the FP64 moment scratch aliases uh");
 };
 
@@ -185,11 +199,12 @@ __global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S>())) void k_stage(
     const double *__restrict__ unodes, // (nu+2)*N
     const double *__restrict__ vnodes, // (nv+2)*N
     StageArgs s, const OpTables *__restrict__ tabp) {
-    __shared__ typename Store<S>::C ft[StageLds<N, S>::n_ft];
+    __shared__ __attribute__((aligned(16))) typename Store<S>::C ft[StageLds<N, S>::n_ft];
+    __shared__ __attribute__((aligned(16))) typename Store<S>::C acc_l[StageLds<N, S>::n_acc];
+    __shared__ typename Store<S>::C vh[StageLds<N, S>::n_vh];
     __shared__ typename Store<S>::C xl[StageLds<N, S>::n_x];
     __shared__ typename Store<S>::C xr[StageLds<N, S>::n_x];
     __shared__ typename Store<S>::C uh[StageLds<N, S>::n_uh];
-    __shared__ typename Store<S>::C acc_l[StageLds<N, S>::n_acc];
     __shared__ double bgs[StageLds<N, S>::n_bgs];
     __shared__ double red_own[StageLds<N, S>::n_red];
     if constexpr (ADT) stage_dt_from_device(s);   // adaptive runs: dt from the device clock
