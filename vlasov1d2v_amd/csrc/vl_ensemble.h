@@ -131,8 +131,6 @@ __device__ __forceinline__ void ens_stage_pencil(const EnsRun &r, int p, int sta
     const Pencil pc = decode_pencil(s, p);
     C *ft = reinterpret_cast<C *>(lds + LY::o_ft);
     C *vh = reinterpret_cast<C *>(lds + LY::o_vh);
-    C *xl = reinterpret_cast<C *>(lds + LY::o_xl);
-    C *xr = reinterpret_cast<C *>(lds + LY::o_xr);
     C *uh = reinterpret_cast<C *>(lds + LY::o_uh);
     C *acc_l = reinterpret_cast<C *>(lds + LY::o_acc);
     double *bgs = reinterpret_cast<double *>(lds + LY::o_bgs);

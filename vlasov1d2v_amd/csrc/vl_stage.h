@@ -147,8 +147,8 @@ __device__ __forceinline__ Pencil decode_pencil(const StageArgs &s, int p) {
 //            element-major layout: node (el, ix, iu, iv) at el EP + (ix N + iu) N + iv,
 //            so that the 16-byte rows of that layout move as whole vectors
 //   vh       v-halo nodes of the tile rows r = ix N + iu: [side][r]
-//   xl, xr   left / right x-halo planes [el][iu][iv]
 //   uh       u-halo faces [side][el][ix][iv]
+// (the x halos stay in registers: one value per x-line and thread)
 // With one line per thread in the order (el, a, b) the x-line reads of ft and
 // the v-line reads (9 consecutive values, lanes 9 apart) are free of LDS bank
 // conflicts on 64 x 4-byte banks, and the epilogue's 16-byte accesses are
@@ -162,7 +162,6 @@ struct StageLds {
     static constexpr int n_ft = TileShape<N>::TV * EP;   // tile values, element-major
     static constexpr int n_acc = n_ft;                   // RHS accumulator, same layout
     static constexpr int n_vh = 2 * N * N;               // v halo [side][ix*N + iu]
-    static constexpr int n_x = N * P;                    // left / right x halo [el][iu][iv]
     static constexpr int n_uh = 2 * N * P;               // u halo [side][el][ix][iv]
     static constexpr int n_bgs = BG ? 3 * N * P : 1;     // background slab [table][iu][q]
     static constexpr int n_red = BG ? N * P : 1;         // moment scratch (the FP64 form reuses uh)
@@ -171,26 +170,22 @@ struct StageLds {
     static constexpr size_t o_bgs = o_acc + (size_t)n_acc * sizeof(C);
     static constexpr size_t o_red = o_bgs + (size_t)n_bgs * 8;
     static constexpr size_t o_vh = o_red + (size_t)n_red * 8;
-    static constexpr size_t o_xl = o_vh + (size_t)n_vh * sizeof(C);
-    static constexpr size_t o_xr = o_xl + (size_t)n_x * sizeof(C);
-    static constexpr size_t o_uh = o_xr + (size_t)n_x * sizeof(C);
+    static constexpr size_t o_uh = o_vh + (size_t)n_vh * sizeof(C);
     static constexpr size_t bytes = o_uh + (size_t)n_uh * sizeof(C);
     static_assert((EP * sizeof(C)) % 16 == 0, "ft / acc elements start 16-byte aligned");
     static_assert(BG || (o_uh % 8 == 0), "the FP64 moment scratch aliases uh");
 };
 
-// Workgroups per CU the stage kernel is compiled for.  FP64 at n = 9 uses 44 KB
-// of LDS, which caps a CU at 3 workgroups whatever is asked; asking for 4
-// changes only the compiler's instruction schedule (registers stay at ~166),
-// and that schedule measured 3.6 % faster on MI355X (0.656 against 0.681 ms
-// per stage-2/3 launch at 116.6M DOF).
-template <int N, int S>
+// Workgroups per CU the stage kernel is compiled for.  The element-major FP64
+// kernel at n = 9 fits 4 per CU (40.5 KB of LDS, 128 registers); the others
+// are compiled for 3 (the reference-layout kernels would spill at 128).
+template <int N, int S, int L>
 constexpr int stage_min_blocks() {
-    return (N == 9 && S == 0) ? 4 : 3;
+    return (N == 9 && S == 0 && L == kElemLayout) ? 4 : 3;
 }
 
 template <int N, int MODE, int S, int L, bool ADT = false>
-__global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S>())) void k_stage(
+__global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S, L>())) void k_stage(
     const typename Store<S>::E *__restrict__ fin,  // stage input (interior; ghosts too when !periodic_x)
     const typename Store<S>::E *__restrict__ fgh,  // array whose u/v ghost planes are used
     const typename Store<S>::E *__restrict__ fn,   // f^n for MODE == kModeStage (may alias fout)
@@ -202,8 +197,6 @@ __global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S>())) void k_stage(
     __shared__ __attribute__((aligned(16))) typename Store<S>::C ft[StageLds<N, S>::n_ft];
     __shared__ __attribute__((aligned(16))) typename Store<S>::C acc_l[StageLds<N, S>::n_acc];
     __shared__ typename Store<S>::C vh[StageLds<N, S>::n_vh];
-    __shared__ typename Store<S>::C xl[StageLds<N, S>::n_x];
-    __shared__ typename Store<S>::C xr[StageLds<N, S>::n_x];
     __shared__ typename Store<S>::C uh[StageLds<N, S>::n_uh];
     __shared__ double bgs[StageLds<N, S>::n_bgs];
     __shared__ double red_own[StageLds<N, S>::n_red];
