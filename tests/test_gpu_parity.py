@@ -264,3 +264,22 @@ def test_resident_layout_choice(cuda):
         if want == _lib.VL_LAYOUT_ROW_SPLIT:
             g.set_tuning(_lib.VL_TUNE_ROW_SPLIT, 0)
             assert g.resident_layout() == _lib.VL_LAYOUT_ELEMENT_MAJOR
+
+
+# every node count, with a partial last v-chunk (nv = TV + 1, TV = 256 // n^2
+# elements per tile) so that the element-major tile's halo and empty-element
+# paths run: n = 2 (TV 64) ... n = 10 (TV 2)
+ORDER_CASES = [(2, 3, 3, 65), (3, 3, 3, 29), (4, 3, 3, 17), (5, 3, 3, 11), (6, 3, 3, 8), (7, 2, 3, 6),
+               (8, 2, 3, 5), (9, 2, 3, 4), (10, 2, 2, 3)]
+
+
+@pytest.mark.parametrize("n,nx,nu,nv", ORDER_CASES)
+def test_resident_steps_every_order_match_oracle(cuda, n, nx, nu, nv):
+    """Resident (element-major) FP64 steps against the oracle for n = 2..10."""
+    case, run = make_pair(n, nx, nu, nv, dt=0.002)
+    st_ref = case.run(3, 0.002)
+    st = run.run(3)
+    err_f = rel_l2(interior(run.dist.arr.cpu().numpy()), interior(case.dist.arr))
+    err_e = rel_l2(st.field_energy, st_ref.field_energy)
+    print(f"n={n}: rel-L2 f {err_f:.2e}, energy {err_e:.2e}", flush=True)
+    assert err_f < 1e-12 and err_e < 1e-12
