@@ -100,6 +100,9 @@ constexpr unsigned kSelPos = 0x05040100u;   // (nbL, f_{n-1}): a >= 0
 constexpr unsigned kSelNeg = 0x03020706u;   // (f_0, nbR):     a <  0
 constexpr unsigned kSelKeep = 0x03020100u;  // d1 unchanged (other lane groups)
 
+typedef __bf16 vl_bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 vl_f16x2 __attribute__((ext_vector_type(2)));
+
 template <int S> struct RsPack;
 template <> struct RsPack<2> {              // BF16
     // two FP32 values that are exact BF16 numbers -> BF16 pair (a in the low half)
@@ -110,14 +113,19 @@ template <> struct RsPack<2> {              // BF16
     __device__ static float lo(unsigned w) { return __builtin_bit_cast(float, w << 16); }
     __device__ static float hi(unsigned w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
     // stochastic rounding: add 16 random bits below the BF16 mantissa and
-    // truncate (rounds up with probability equal to the discarded fraction)
+    // truncate (rounds up with probability equal to the discarded fraction).
+    // gfx950's v_cvt_sr_bf16_f32 does exactly that with the HIGH 16 bits of
+    // its random operand: a takes the low half of r, b the high half.
     __device__ static unsigned sr2(float a, float b, unsigned r) {
-        const unsigned ua = __builtin_bit_cast(unsigned, a) + (r & 0xffffu);
-        const unsigned ub = __builtin_bit_cast(unsigned, b) + (r >> 16);
-        return __builtin_amdgcn_perm(ub, ua, 0x07060302u);
+        vl_bf16x2 v = {};
+        v = __builtin_amdgcn_cvt_sr_bf16_f32(v, a, r << 16, false);
+        v = __builtin_amdgcn_cvt_sr_bf16_f32(v, b, r, true);
+        return __builtin_bit_cast(unsigned, v);
     }
     __device__ static unsigned sr1(float a, unsigned r) {
-        return (__builtin_bit_cast(unsigned, a) + (r & 0xffffu)) >> 16;
+        vl_bf16x2 v = {};
+        v = __builtin_amdgcn_cvt_sr_bf16_f32(v, a, r << 16, false);
+        return __builtin_bit_cast(unsigned, v) & 0xffffu;
     }
 };
 template <> struct RsPack<3> {              // FP16
@@ -126,17 +134,22 @@ template <> struct RsPack<3> {              // FP16
     }
     __device__ static float lo(unsigned w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
     __device__ static float hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
-    // stochastic rounding: uniform noise of one FP16 spacing (the spacing at
-    // a's binade, 2^-24 in the subnormal range), then round to nearest
-    __device__ static float dither(float a, unsigned r16) {
-        const unsigned e = (__builtin_bit_cast(unsigned, a) >> 23) & 0xffu;
-        const float ulp = __builtin_bit_cast(float, ((e < 113u ? 113u : e) - 10u) << 23);
-        return a + ((float)r16 * (1.0f / 65536.0f) - 0.5f) * ulp;
-    }
+    // stochastic rounding with gfx950's v_cvt_sr_f16_f32: for a normal FP16
+    // result it adds the top 13 bits of the random operand below the FP16
+    // mantissa and truncates (rounds up with probability equal to the
+    // discarded fraction; also unbiased in the subnormal range).  a takes
+    // bits 31..19 of r, b bits 18..6.
     __device__ static unsigned sr2(float a, float b, unsigned r) {
-        return MxFmt<3>::enc2(dither(a, r & 0xffffu), dither(b, r >> 16));
+        vl_f16x2 v = {};
+        v = __builtin_amdgcn_cvt_sr_f16_f32(v, a, r, false);
+        v = __builtin_amdgcn_cvt_sr_f16_f32(v, b, r << 13, true);
+        return __builtin_bit_cast(unsigned, v);
     }
-    __device__ static unsigned sr1(float a, unsigned r) { return MxFmt<3>::enc(dither(a, r & 0xffffu)); }
+    __device__ static unsigned sr1(float a, unsigned r) {
+        vl_f16x2 v = {};
+        v = __builtin_amdgcn_cvt_sr_f16_f32(v, a, r, false);
+        return __builtin_bit_cast(unsigned, v) & 0xffffu;
+    }
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
