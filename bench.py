@@ -44,9 +44,26 @@ def cpu_baseline(w, seconds_target=15.0):
         if time.perf_counter() - t0 > seconds_target or steps >= 50:
             break
     dt = time.perf_counter() - t0
-    return {"value": dof * steps / dt, "unit": "DOF-updates/s", "cores": int(threads), "kind": "port",
-            "sample": f"NumPy FP64 oracle, {steps} SSP-RK3 steps of the same workload with NX={nx} "
-                      f"instead of {w.nx} ({dof} DOF), {dt:.1f} s"}
+    out = {"value": dof * steps / dt, "unit": "DOF-updates/s", "cores": int(threads), "kind": "port",
+           "sample": f"NumPy FP64 oracle, {steps} SSP-RK3 steps of the same workload with NX={nx} "
+                     f"instead of {w.nx} ({dof} DOF), {dt:.1f} s"}
+    # the same sample on one thread (BLAS pools limited to 1), a few seconds
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            steps1 = 0
+            while True:
+                st.nonlinear_ssp_rk(case.dist, case.basis, case.elliptic, case.grids, case.flux)
+                steps1 += 1
+                if time.perf_counter() - t0 > seconds_target / 3 or steps1 >= 20:
+                    break
+            dt1 = time.perf_counter() - t0
+        out["single_thread"] = {"value": dof * steps1 / dt1, "cores": 1,
+                                "sample": f"{steps1} steps of the same sample, {dt1:.1f} s"}
+    except Exception:
+        pass
+    return out
 
 
 def main():
