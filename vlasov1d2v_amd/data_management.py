@@ -68,17 +68,17 @@ def _check_io(rc, what):
 
 
 def _host_f32(a):
-    """A small array (field, density) as contiguous float32 on the host."""
+    """A float32 host copy of an array (the writer thread may use it after the
+    caller has changed the original)."""
     try:
         import torch
         if isinstance(a, torch.Tensor):
-            a = a.detach().to("cpu")
-            return np.ascontiguousarray(a.numpy(), dtype=np.float32)
+            a = a.detach().to("cpu").numpy()
     except ImportError:
         pass
     if hasattr(a, "get"):
         a = a.get()
-    return np.ascontiguousarray(np.asarray(a), dtype=np.float32)
+    return np.array(a, dtype=np.float32, order="C", copy=True)
 
 
 def _state_geometry(distribution):
@@ -146,8 +146,7 @@ class RunData:
         if isinstance(distribution, torch.Tensor):
             if distribution.is_cuda:
                 raise ValueError("pass the Distribution (GPU state) or a host array")
-            distribution = distribution.numpy()
-        a = np.ascontiguousarray(np.asarray(distribution), dtype=np.float32)
+        a = _host_f32(distribution)
         if a.shape != self.shape:
             raise ValueError(f"state shape {a.shape} != file shape {self.shape}")
         return a, None, None
