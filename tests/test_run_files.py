@@ -203,3 +203,21 @@ def test_io_library_exports_every_header_symbol():
     for s in syms:
         assert hasattr(lib, s), s
     assert shutil.which("gcc") is not None
+
+
+def test_async_write_keeps_the_values_at_the_call(tmp_path):
+    """save_data returns before the file is written; changing the caller's
+    arrays afterwards must not change the record."""
+    _need_io()
+    pdf, pot, den = _records(2, seed=11)
+    rd = dm.RunData(str(tmp_path) + os.sep, "run", SHAPE, None, None)
+    rd.create_file(pdf[0], _Field(pot[0]), den[0])
+    a = pdf[1].astype(np.float32)
+    p = pot[1].copy()
+    rd.save_data(a, _Field(p), den[1], 1.0, 2.0)
+    a[...] = -1.0
+    p[...] = -1.0
+    rd.close()
+    t, got, gp, _, _ = dm.ReadData(str(tmp_path) + os.sep, "run").read_data()
+    np.testing.assert_array_equal(got[1], pdf[1].astype(np.float32))
+    np.testing.assert_array_equal(gp[1], pot[1].astype(np.float32))
