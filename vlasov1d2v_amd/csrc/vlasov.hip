@@ -632,12 +632,13 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         return fail(VL_E_ARG, "grid too large");
     }
     h->tiles = (int)tiles;
-    // pencils of the stage kernel: about 13000 workgroups (measured best on
-    // MI355X for the 116.6M-DOF grid: 4 x elements per pencil); host-tunable
-    // with vl_set_tuning
+    // pencils of the stage kernel: about 6500 workgroups (measured best on
+    // MI355X for the 116.6M-DOF grid among 2, 4, 8 and 16 x elements per
+    // pencil: 8, with 4 workgroups per CU for FP64); host-tunable with
+    // vl_set_tuning
     {
         const int base = d->nu * h->nchunk;
-        int nseg = (13000 + base - 1) / base;
+        int nseg = (6500 + base - 1) / base;
         nseg = std::max(1, std::min(nseg, d->nx));
         h->xseg = (d->nx + nseg - 1) / nseg;
         h->mfma = 1;
