@@ -275,6 +275,19 @@ int vl_ensemble_bind(vl_ensemble_t e, int run, void *f, void *f1, void *f2, doub
 int vl_ensemble_set_schedule(vl_ensemble_t e, int work_per_item, int grid, int order);
 int vl_ensemble_get_schedule(vl_ensemble_t e, int *work_per_item, int *grid, int *order, int *items);
 
+/* Adaptive time step for one run of an ensemble (main/timestep.py:184,
+ * 231-242, as vl_rk3_step_adaptive does for a single run): the run's dt and
+ * time live in the clock of the workspace it was bound with; the stage-3
+ * field solve forms its next dt and one kernel after the step makes it the
+ * step's dt and advances the time.  Needs vl_set_cfl on the run's geometry
+ * and vl_clock_init (with the set-up field) on its workspace; the dt given to
+ * vl_ensemble_bind is then ignored.  on = 0 returns the run to its fixed dt. */
+int vl_ensemble_set_adaptive(vl_ensemble_t e, int run, int on);
+
+/* (time, dt) of every run's clock into out[2 i], out[2 i + 1] (device, FP64;
+ * zeros for fixed-dt runs), on `stream`. */
+int vl_ensemble_clocks(vl_ensemble_t e, double *out, void *stream);
+
 /* Moment partials of every run's f: call once before the first step of a
  * loop (a step leaves those of its result for the next one). */
 int vl_ensemble_moments(vl_ensemble_t e, void *stream);

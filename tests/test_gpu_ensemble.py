@@ -132,3 +132,30 @@ def _step_energy(ens):
     e = torch.empty(len(ens.runs), dtype=torch.float64, device=ens.device)
     ens.step(e)
     return e.cpu().numpy()
+
+
+@pytest.mark.parametrize("graph_steps", [0, 2])
+def test_ensemble_adaptive_dt_matches_single_device_dt_loops(cuda, graph_steps):
+    """dt="adaptive": every run takes the CFL step of its own field after each
+    step (main/timestep.py:184,231-242), as a single run's device-dt loop does."""
+    import torch
+    from vlasov1d2v_amd import Ensemble, Run
+    steps = 5
+    members = _members()
+    ens = Ensemble(members, storage="f64", device=cuda, dt="adaptive")
+    energies = ens.run(steps, graph_steps=graph_steps)
+    clocks = ens.clocks()
+    ens.finish()
+    torch.cuda.synchronize()
+    for r, m in enumerate(members):
+        run = Run(m.n, m.nx, m.nu, m.nv, k0=m.k0, b_field=m.b_field, ring_j=m.ring_j, seed=m.seed, dt=None)
+        st = run.stepper
+        st.main_loop(run.dist, run.basis, run.elliptic, run.grids, run.flux, steps=steps, device_dt=True)
+        err_e = rel_l2(energies[:, r], np.asarray(st.field_energy))
+        err_f = rel_l2(interior(ens.runs[r].dist.total().cpu().numpy()), interior(run.dist.total().cpu().numpy()))
+        print(f"member {m}: time {clocks[r, 0]!r} vs {st.time!r}, dt {clocks[r, 1]!r} vs {st.dt!r}, "
+              f"rel-L2 f {err_f:.2e}, energy {err_e:.2e}", flush=True)
+        assert abs(clocks[r, 0] - st.time) <= 1e-12 * st.time
+        assert abs(clocks[r, 1] - st.dt) <= 1e-12 * st.dt
+        assert clocks[r, 1] != ens.dts[r]           # the step changed with the field
+        assert err_f < 1e-12 and err_e < 1e-12, (m, err_f, err_e)

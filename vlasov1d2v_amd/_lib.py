@@ -24,7 +24,7 @@ EXPORTS = (
     "vl_ensemble_create", "vl_ensemble_destroy", "vl_ensemble_bind", "vl_ensemble_set_schedule",
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
-    "vl_poisson_central", "vl_export_f32", "vl_import_f32",
+    "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -106,6 +106,8 @@ def load(path=LIB_PATH):
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
         "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
         "vl_ensemble_destroy": [vp],
+        "vl_ensemble_set_adaptive": [vp, i32, i32],
+        "vl_ensemble_clocks": [vp, vp, vp],
         "vl_ensemble_bind": [vp, i32, vp, vp, vp, vp, vp, f64],
         "vl_ensemble_set_schedule": [vp, i32, i32, i32],
         "vl_ensemble_get_schedule": [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),

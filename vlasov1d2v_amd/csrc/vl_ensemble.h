@@ -40,9 +40,14 @@ struct EnsRun {
     double2 *coef, *enpart;
     double *mxpart;
     unsigned *ticket, *seed;
-    FieldArgs fa;                  // from_rho = 1
+    FieldArgs fa;                  // from_rho = 1 (CFL fields set for adaptive runs)
     int64_t plane;                 // values per x element of the element-major state
     int n, tpe, inv_blocks, swap_blocks;
+    // adaptive time step (main/timestep.py:184,231-242): the run's workspace
+    // clock (dt, time, next dt) and the stage factors c2 of c2 * dt
+    double *clock;
+    double c2[3];
+    int adaptive;
 };
 
 // workgroup -> (run, local workgroup) of the field kernels
@@ -84,16 +89,42 @@ __global__ __launch_bounds__(kBlock) void k_ens_forward(const EnsRun *__restrict
     dft_forward_body(r.rho, r.T, r.kwave, r.coef, r.fa, b.local, dyn, sh, sl);
 }
 
-// energy: null, or one value per run (written for the run's field)
+// energy: null, or one value per run (written for the run's field).
+// last != 0 (stage 3): adaptive runs also write their next dt from this
+// field (the field the reference's adapt_time_step reads after the step).
 __global__ __launch_bounds__(kBlock) void k_ens_inverse(const EnsRun *__restrict__ runs,
                                                         const EnsBlock *__restrict__ map,
-                                                        double *__restrict__ energy) {
+                                                        double *__restrict__ energy, int last) {
     extern __shared__ double2 cdyn[];                 // max 2*M over the runs
     __shared__ InvLds ls;
     const EnsBlock b = map[blockIdx.x];
     const EnsRun &r = runs[b.run];
+    FieldArgs fa = r.fa;
+    fa.dt_next = (last && r.adaptive) ? r.clock + 2 : nullptr;
     dft_inverse_body(r.coef, r.phases, r.wts, r.E, nullptr, r.enpart, r.mxpart, r.ticket,
-                     energy ? energy + b.run : nullptr, nullptr, r.fa, b.local, r.inv_blocks, cdyn, ls);
+                     energy ? energy + b.run : nullptr, nullptr, fa, b.local, r.inv_blocks, cdyn, ls);
+}
+
+// After a step of every run: adaptive runs take the next dt and advance their
+// time by it (main/timestep.py:96,184), as k_clock_advance does for one run.
+__global__ void k_ens_clock_advance(const EnsRun *__restrict__ runs, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        if (!runs[i].adaptive) continue;
+        double *c = runs[i].clock;
+        const double dt = c[2];
+        const double t = c[1] + dt;
+        c[0] = dt;
+        c[1] = t;
+    }
+}
+
+// (time, dt) of every run's clock -> out[2 i], out[2 i + 1] (fixed-dt runs: 0, 0)
+__global__ void k_ens_clocks(const EnsRun *__restrict__ runs, int count, double *__restrict__ out) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const bool a = runs[i].adaptive != 0;
+        out[2 * i] = a ? runs[i].clock[1] : 0.0;
+        out[2 * i + 1] = a ? runs[i].clock[0] : 0.0;
+    }
 }
 
 // LDS of the ensemble stage kernel: the largest single-run layout
@@ -118,7 +149,7 @@ __device__ __forceinline__ void ens_stage_pencil(const EnsRun &r, int p, int sta
     StageArgs s = r.s;
     s.c0 = r.cf[stage - 1][0];
     s.c1 = r.cf[stage - 1][1];
-    s.c2dt = r.cf[stage - 1][2];
+    s.c2dt = r.adaptive ? r.c2[stage - 1] * r.clock[0] : r.cf[stage - 1][2];   // c2 * dt as the host forms it
     T *fs = static_cast<T *>(r.f);
     const T *fin = stage == 1 ? fs : static_cast<const T *>(stage == 2 ? r.f1 : r.f2);
     const T *fgh = fs;

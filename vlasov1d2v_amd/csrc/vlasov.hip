@@ -1261,6 +1261,7 @@ struct vl_ensemble {
     int nb_swap, nb_red, nb_fwd, nb_inv;
     size_t fwd_lds, inv_lds;
     bool dirty;                    // host tables changed since the last upload
+    int n_adaptive;                // runs with the adaptive time step
 };
 
 namespace {
@@ -1344,8 +1345,9 @@ hipError_t ens_stage(const vl_ensemble *e, int stage, hipStream_t st) {
     return e->storage == VL_F64 ? ens_stage_launch<VL_F64>(e, stage, st) : ens_stage_launch<VL_F32>(e, stage, st);
 }
 
-// moment partials -> rho -> E of every run (energy: one value per run, or null)
-hipError_t ens_field(const vl_ensemble *e, double *energy, hipStream_t st) {
+// moment partials -> rho -> E of every run (energy: one value per run, or
+// null; last: the stage-3 solve, where adaptive runs form their next dt)
+hipError_t ens_field(const vl_ensemble *e, double *energy, bool last, hipStream_t st) {
     hipLaunchKernelGGL(k_ens_reduce, dim3(e->nb_red), dim3(kBlock), 0, st, e->d_runs, e->d_map_red);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
@@ -1353,7 +1355,7 @@ hipError_t ens_field(const vl_ensemble *e, double *energy, hipStream_t st) {
     err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_ens_inverse, dim3(e->nb_inv), dim3(kBlock), e->inv_lds, st, e->d_runs, e->d_map_inv,
-                       energy);
+                       energy, last ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1368,8 +1370,12 @@ hipError_t ens_swap(const vl_ensemble *e, hipStream_t st) {
 hipError_t ens_step(const vl_ensemble *e, double *energy, hipStream_t st) {
     hipError_t err = ens_swap(e, st);
     for (int stage = 1; stage <= 3 && err == hipSuccess; ++stage) {
-        err = ens_field(e, stage == 3 ? energy : nullptr, st);
+        err = ens_field(e, stage == 3 ? energy : nullptr, stage == 3, st);
         if (err == hipSuccess) err = ens_stage(e, stage, st);
+    }
+    if (err == hipSuccess && e->n_adaptive > 0) {
+        hipLaunchKernelGGL(k_ens_clock_advance, dim3(1), dim3(kBlock), 0, st, e->d_runs, e->count);
+        err = hipGetLastError();
     }
     return err;
 }
@@ -1492,12 +1498,42 @@ int vl_ensemble_bind(vl_ensemble_t e, int run, void *f, void *f1, void *f2, doub
     x.mxpart = ws_mxpart(h, ws);
     x.ticket = ws_ticket(h, ws);
     x.seed = ws_seed(h, ws);
+    x.clock = ws_clock(h, ws);
+    x.c2[0] = 1.0;
+    x.c2[1] = 1.0 / 4.0;
+    x.c2[2] = 2.0 / 3.0;
     // the Shu-Osher coefficients as vl_rk3_step forms them (main/timestep.py:37-38)
     const double cf[3][3] = {{0.0, 1.0, dt}, {3.0 / 4.0, 1.0 / 4.0, (1.0 / 4.0) * dt},
                              {1.0 / 3.0, 2.0 / 3.0, (2.0 / 3.0) * dt}};
     std::memcpy(x.cf, cf, sizeof(cf));
     e->bound[run] = 1;
     e->dirty = true;
+    return VL_OK;
+}
+
+int vl_ensemble_set_adaptive(vl_ensemble_t e, int run, int on) {
+    if (int r = ens_check(e)) return r;
+    if (run < 0 || run >= e->count) return fail(VL_E_ARG, "vl_ensemble_set_adaptive: run %d outside [0,%d)", run, e->count);
+    if (!e->bound[run]) return fail(VL_E_ARG, "vl_ensemble_set_adaptive: run %d has no buffers (vl_ensemble_bind)", run);
+    const vl_geom *h = e->geoms[run];
+    EnsRun &x = e->runs[run];
+    if (on && !h->cfl_set) return fail(VL_E_ARG, "vl_ensemble_set_adaptive: run %d needs vl_set_cfl first", run);
+    if (on) set_cfl_args(h, x.fa);
+    x.fa.dt_next = nullptr;                 // set per launch for the stage-3 solve
+    const int was = x.adaptive;
+    x.adaptive = on ? 1 : 0;
+    e->n_adaptive += x.adaptive - was;
+    e->dirty = true;
+    return VL_OK;
+}
+
+int vl_ensemble_clocks(vl_ensemble_t e, double *out, void *stream) {
+    if (int r = ens_check(e)) return r;
+    if (!out) return fail(VL_E_NULL, "vl_ensemble_clocks: null pointer");
+    if (!aligned(out, 8)) return fail(VL_E_ALIGN, "vl_ensemble_clocks: misaligned pointer");
+    if (int r = ens_ready(e, "vl_ensemble_clocks")) return r;
+    hipLaunchKernelGGL(k_ens_clocks, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, e->d_runs, e->count, out);
+    VL_HIP(hipGetLastError());
     return VL_OK;
 }
 

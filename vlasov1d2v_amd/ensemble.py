@@ -75,9 +75,11 @@ def cfl_dt(run):
 class Ensemble:
     """Members stepped together; states stay resident (element-major) until ``finish()``.
 
-    ``dt``: one fixed step for every run, a sequence of one per run, or None
+    ``dt``: one fixed step for every run, a sequence of one per run, None
     for 0.85 x each run's CFL step at t = 0 (the margin the single-run
-    workloads keep below the rule).
+    workloads keep below the rule), or "adaptive" for the reference's CFL
+    rule after every step of every run (main/timestep.py:184,231-242),
+    evaluated on the device per run (``clocks()`` reads each run's time and dt).
     """
 
     def __init__(self, members, storage="f32", device="cuda", dt=None, safety=0.85):
@@ -89,10 +91,13 @@ class Ensemble:
         self.device = torch.device(device)
         self.runs = []
         self.dts = []
+        self.adaptive = isinstance(dt, str) and dt == "adaptive"
         for m in self.members:
             run = Run(m.n, m.nx, m.nu, m.nv, k0=m.k0, b_field=m.b_field, ring_j=m.ring_j, seed=m.seed,
                       dt=1.0, device=self.device, storage=storage)
-            if dt is None:
+            if self.adaptive:
+                step = cfl_dt(run)          # the rule at t = 0 (the clock starts from it)
+            elif dt is None:
                 step = safety * cfl_dt(run)
             elif np.ndim(dt) == 0:
                 step = float(dt)
@@ -117,6 +122,15 @@ class Ensemble:
             _lib.check(lib.vl_ensemble_bind(h, r, _lib.ptr(f), _lib.ptr(f1), _lib.ptr(f2),
                                             _lib.ptr(run.elliptic.electric_field), _lib.ptr(ws),
                                             self.dts[r]), "vl_ensemble_bind")
+        if self.adaptive:
+            for r, run in enumerate(self.runs):
+                g = self._geoms[r]
+                gr = run.grids
+                g.set_cfl(run.stepper.courant, gr.x.dx, gr.u.dx, gr.v.dx)
+                _lib.check(lib.vl_clock_init(g.handle, _lib.ptr(run.elliptic.electric_field), 0.0,
+                                             _lib.ptr(g.workspace(self.device)), _lib.stream_ptr()),
+                           "vl_clock_init")
+                _lib.check(lib.vl_ensemble_set_adaptive(h, r, 1), "vl_ensemble_set_adaptive")
         _lib.check(lib.vl_ensemble_moments(h, _lib.stream_ptr()), "vl_ensemble_moments")
         self.steps_done = 0
         self.tuning = None
@@ -152,6 +166,14 @@ class Ensemble:
         _lib.check(_lib.load().vl_ensemble_rk3_step(self.handle, _lib.ptr(energy_out), _lib.stream_ptr()),
                    "vl_ensemble_rk3_step")
         self.steps_done += 1
+
+    def clocks(self):
+        """(time, dt) of every adaptive run after the steps so far, (runs, 2) on the host."""
+        import torch
+        out = torch.zeros((len(self.runs), 2), dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().vl_ensemble_clocks(self.handle, _lib.ptr(out), _lib.stream_ptr()),
+                   "vl_ensemble_clocks")
+        return out.cpu().numpy()
 
     def capture(self, steps):
         """A HIP graph of ``steps`` ensemble steps (step once eagerly first)."""
