@@ -63,6 +63,16 @@ __device__ __forceinline__ vl_u32x4 buf_ld16(__amdgpu_buffer_rsrc_t r, int off) 
 __device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, int off, vl_u32x4 v) {
     __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
 }
+// Streaming forms (cache policy NT): for data a kernel touches once -- the
+// stage output and f^n -- so that they do not push the re-read stage input
+// (halo rows of neighbouring tiles) out of the caches.
+constexpr int kCachePolicyNT = 2;
+__device__ __forceinline__ vl_u32x4 buf_ld16_nt(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kCachePolicyNT);
+}
+__device__ __forceinline__ void buf_st16_nt(__amdgpu_buffer_rsrc_t r, int off, vl_u32x4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, kCachePolicyNT);
+}
 
 // Storage traits of the distribution function: element type E in HBM,
 // compute type C, whether the stored value is the deviation g = (f - f0) *

@@ -387,14 +387,14 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             const int bfn = (int)(ebase(ex, eu, ev) * 2);
             unsigned *st0 = fnst + el * FNW;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)st0, 16,
-                                                     bfn + ln * 16, 0, 0, 0);
+                                                     bfn + ln * 16, 0, 0, kCachePolicyNT);
             if (ln < ROWS - 64)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + 256),
-                                                         16, bfn + (64 + ln) * 16, 0, 0, 0);
+                                                         16, bfn + (64 + ln) * 16, 0, 0, kCachePolicyNT);
             if (N == 9)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + FNR),
                                                          4, ln < (ROWS + 1) / 2 ? bfn + 16 * ROWS + 4 * ln : kOob,
-                                                         0, 0, 0);
+                                                         0, 0, kCachePolicyNT);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (has_next) load(ex + 1, nxt);
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                         w.z = pair(l1[0], l1[1], f23.x, nw[2], m, 4);
                         w.w = pair(l1[2], l1[3], f23.y, nw[3], m, 6);
                     }
-                    buf_st16(rout, (int)(bo * 2) + r * 16, w);
+                    buf_st16_nt(rout, (int)(bo * 2) + r * 16, w);
                     if (N == 9) {
                         const float fi8 = PK::lo(br[4]);
                         const float fn8 = (r & 1) ? PK::hi(fnt) : PK::lo(fnt);
