@@ -46,8 +46,14 @@ Pinning (tests/test_oracle_*.py):
                           operator (Lagrange-quadrature weak form) applied to
                           unit vectors on tiny grids, plus free-stream,
                           conservation and spectral-convergence checks.
+* ``lgl``              -- also the decimal tables of main/basis.py:9-57
+                          (tests/golden/lgl_literals.json) and their closed
+                          forms, to a few ulp.
 * ``Stepper``          -- SSP-RK3 amplification polynomial on a linear
-                          problem, Shu-Osher table.
+                          problem, Shu-Osher table; on the coupled
+                          Vlasov-Poisson system third-order temporal
+                          self-convergence (a stepper that reuses the f^n
+                          field for every stage measures first order).
 * ``electric_energy``  -- closed form for E = A cos(kx).
 * ``eigenfunction``    -- the Jacobi-Anger sum exp(-i b sin t) = sum_n J_n(b)
                           exp(-i n t) at om = 0, and independent Bessel values
@@ -65,8 +71,12 @@ Pinning (tests/test_oracle_*.py):
                           grid and field) and the zero-field closed form; the
                           loop order (adapt after the step, time += new dt)
                           follows the reference text.
-The composite conventions (frozen u/v ghosts, lagged energy series) follow
-the reference text; no reference-held vectors exist to pin them further.
+* ``Stepper.main_loop`` energy series -- entry k >= 1 equals the energy of
+                          the field solved from f2 recomputed independently
+                          with the pinned flux and Poisson solve, and differs
+                          from the energy of a re-solve of f^{n+1}.
+The frozen u/v ghost planes follow the reference text (main/timestep.py:
+148-151,187); no reference-held vectors exist to pin them further.
 """
 import math
 
@@ -598,12 +608,13 @@ class Stepper:
         distribution.arr[ng] += c[-1] * self.dt * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
 
     def main_loop(self, steps, distribution, basis, elliptic, grids, dg_flux, courant=None, acc=None,
-                  final_time=None):  # main/timestep.py:81-139
+                  final_time=None, on_step=None):  # main/timestep.py:81-139
         """Fixed dt, or with ``courant`` the reference's adaptive rule: dt from
         the set-up field before the loop (main/timestep.py:88-89), again after
         every step from the field of the last stage solve (:184), and the time
         advanced by that new dt (:96).  ``final_time``: loop while time <
-        final_time instead of ``steps`` steps."""
+        final_time instead of ``steps`` steps.  ``on_step(self)`` runs after
+        every step (the fingerprint writer records checkpoints with it)."""
         self.field_energy = np.append(self.field_energy, elliptic.electric_energy(grids.x))
         adapt = (lambda: adapt_dt(courant, get_max_speeds(grids, elliptic, acc), grids.x.dx, grids.u.dx,
                                   grids.v.dx)) if courant is not None else None
@@ -617,6 +628,8 @@ class Stepper:
             self.steps_counter += 1
             self.time_array = np.append(self.time_array, self.time)
             self.field_energy = np.append(self.field_energy, elliptic.electric_energy(grids.x))
+            if on_step is not None:
+                on_step(self)
 
 
 class Case:

@@ -120,3 +120,45 @@ def test_mass_advection_and_derivative_matrices(n):
     assert np.max(np.abs(b.adv - (D.T * b.weights[None, :]))) < 1e-12 * max(1.0, np.max(np.abs(D)))
     assert np.array_equal(b.stf, b.adv.T)
     assert np.array_equal(b.face_mass, np.eye(n)[:, [0, -1]])
+
+
+def _literals():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lgl_literals.json")
+    with open(path) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("n", [7, 8, 9, 10])
+def test_lgl_matches_reference_literals(n):
+    """The oracle's Newton-iterated Lobatto nodes and weights equal the decimal
+    tables of main/basis.py:9-57 (fixture tests/golden/lgl_literals.json,
+    written by make_lgl_fixture.py from the reference's text) to a few ulp."""
+    lit = _literals()
+    nodes = np.array([float(t) for t in lit["nodes"][str(n)]])
+    weights = np.array([float(t) for t in lit["weights"][str(n)]])
+    x, w = oracle.lgl(n)
+    assert np.max(np.abs(x - nodes)) <= 4.5e-16
+    assert np.max(np.abs(w - weights)) <= 2e-16
+
+
+CLOSED_FORMS = {
+    # Lobatto nodes/weights written as closed forms in main/basis.py:9-57
+    # (textbook values: Abramowitz & Stegun 25.4.32)
+    2: ([-1.0, 1.0], [1.0, 1.0]),
+    3: ([-1.0, 0.0, 1.0], [1 / 3, 4 / 3, 1 / 3]),
+    4: ([-1.0, -np.sqrt(1 / 5), np.sqrt(1 / 5), 1.0], [1 / 6, 5 / 6, 5 / 6, 1 / 6]),
+    5: ([-1.0, -np.sqrt(3 / 7), 0.0, np.sqrt(3 / 7), 1.0], [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10]),
+    6: ([-1.0, -np.sqrt(1 / 3 + 2 * np.sqrt(7) / 21), -np.sqrt(1 / 3 - 2 * np.sqrt(7) / 21),
+         np.sqrt(1 / 3 - 2 * np.sqrt(7) / 21), np.sqrt(1 / 3 + 2 * np.sqrt(7) / 21), 1.0],
+        [1 / 15, (14 - np.sqrt(7)) / 30, (14 + np.sqrt(7)) / 30, (14 + np.sqrt(7)) / 30,
+         (14 - np.sqrt(7)) / 30, 1 / 15]),
+}
+
+
+@pytest.mark.parametrize("n", sorted(CLOSED_FORMS))
+def test_lgl_matches_closed_forms(n):
+    x, w = oracle.lgl(n)
+    assert np.max(np.abs(x - np.array(CLOSED_FORMS[n][0]))) <= 2.3e-16
+    assert np.max(np.abs(w - np.array(CLOSED_FORMS[n][1]))) <= 2.3e-16

@@ -142,3 +142,112 @@ def test_linear_ssp_rk3_amplification_polynomial(z):
     R = 1 + z + z * z / 2 + z ** 3 / 6
     assert np.allclose(interior(case.dist.arr), R * interior(f0), rtol=1e-14, atol=0)
     assert oracle.SSP_RK_LINEAR[3] == [1 / 3, 1 / 2, 1 / 6]
+
+
+# ---- coupled Vlasov-Poisson stepping: temporal self-convergence -------------
+
+class _StaleFieldStepper(oracle.Stepper):
+    """A planted mistake: every stage advances with the field solved from f^n
+    instead of the field of its own input (main/timestep.py:153-182 solves
+    rho -> E before each of the three right-hand sides)."""
+
+    def nonlinear_ssp_rk(self, distribution, basis, elliptic, grids, dg_flux):
+        distribution.swap_ghost_cells_of_array()
+        density = distribution.moment_zero()
+        elliptic.poisson2(np.mean(density) - density, grids.x, basis.b1)
+        ng = grids.no_ghost_slice
+        f = distribution.arr
+        g = f.copy()
+        g[ng] = f[ng] + self.dt * dg_flux.semi_discrete_rhs(f, elliptic, basis, grids)[ng]
+        for i in range(1, self.time_order):
+            g[0] = g[-2]
+            g[-1] = g[1]
+            c = self.coefficients[i - 1]
+            h = g.copy()
+            h[ng] = c[0] * f[ng] + c[1] * g[ng] + c[2] * self.dt * dg_flux.semi_discrete_rhs(
+                g, elliptic, basis, grids)[ng]
+            g = h
+        f[ng] = g[ng]
+
+
+# A strongly coupled small case: a 10 % density modulation at k0 = 0.5 drives
+# a field of O(0.2) that oscillates at about the plasma frequency.
+_COUPLED = dict(n=4, nx=4, nu=6, nv=6, k0=0.5, amp1=0.1, amps=(0.0, 0.0, 0.0), B=0.1)
+
+
+def _final_state(stepper_cls, steps, t_end=1.0):
+    case = oracle.Case(**_COUPLED)
+    st = stepper_cls(t_end / steps)
+    st.main_loop(steps, case.dist, case.basis, case.elliptic, case.grids, case.flux)
+    return interior(case.dist.arr).copy()
+
+
+def _convergence_orders(stepper_cls):
+    """log2 error ratios of dt = 1/32, 1/64, 1/128 against dt = 1/512 at t = 1."""
+    ref = _final_state(stepper_cls, 512)
+    errs = [np.linalg.norm(_final_state(stepper_cls, m) - ref) / np.linalg.norm(ref) for m in (32, 64, 128)]
+    return [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+
+
+def test_coupled_rk3_is_third_order_in_time():
+    """Shu-Osher SSP-RK3 on the coupled Vlasov-Poisson system converges at
+    third order in dt only if each stage solves its field from its own input
+    (main/timestep.py:153-182); the error ratios per halving are ~8."""
+    orders = _convergence_orders(oracle.Stepper)
+    assert all(2.8 < p < 3.3 for p in orders), orders
+
+
+def test_coupled_convergence_detects_stale_stage_field():
+    """The same measurement on the planted stale-field stepper gives first
+    order, so the test above fails for that mistake."""
+    orders = _convergence_orders(_StaleFieldStepper)
+    assert all(p < 1.5 for p in orders), orders
+
+
+# ---- the lagged energy series -----------------------------------------------
+
+def _field_energy_of(case, arr):
+    """Energy of the field solved from a stage array (x ghosts refreshed)."""
+    a = arr.copy()
+    a[0] = a[-2]
+    a[-1] = a[1]
+    rho = case.dist.moment_zero_of_arr(a)
+    ell = oracle.Elliptic(case.elliptic.poisson_coefficient)
+    ell.set_magnetic_field(case.elliptic.magnetic_field)
+    ell.poisson2(np.mean(rho) - rho, case.grids.x, case.basis.b1)
+    return ell, ell.electric_energy(case.grids.x)
+
+
+def test_energy_series_holds_the_stage3_field_energy():
+    """After a step the series appends 0.5 sum w E^2 of the field solved from
+    f2, the input of the third stage (main/timestep.py:102 reads the elliptic
+    state the stage-3 solve left, :170-172), not of the field of f^{n+1}.
+    f1 and f2 are recomputed here with the oracle's flux and Poisson solve
+    (pinned on their own by test_oracle_flux / test_oracle_poisson)."""
+    case = oracle.Case(**_COUPLED)
+    dt = 1.0 / 32
+    ng = case.grids.no_ghost_slice
+    f = case.dist.arr.copy()
+    f[0] = f[-2]
+    f[-1] = f[1]
+    e0 = case.elliptic.electric_energy(case.grids.x)     # set-up solve (main/main.py:131)
+    ell, _ = _field_energy_of(case, f)
+    f1 = f.copy()
+    f1[ng] = f[ng] + dt * case.flux.semi_discrete_rhs(f, ell, case.basis, case.grids)[ng]
+    ell, _ = _field_energy_of(case, f1)
+    f1g = f1.copy()
+    f1g[0] = f1g[-2]
+    f1g[-1] = f1g[1]
+    f2 = f.copy()
+    f2[ng] = 0.75 * f[ng] + 0.25 * f1g[ng] + 0.25 * dt * case.flux.semi_discrete_rhs(
+        f1g, ell, case.basis, case.grids)[ng]
+    _, w_f2 = _field_energy_of(case, f2)
+
+    st = oracle.Stepper(dt)
+    st.main_loop(1, case.dist, case.basis, case.elliptic, case.grids, case.flux)
+    _, w_new = _field_energy_of(case, case.dist.arr)
+    assert st.field_energy[0] == e0
+    assert abs(st.field_energy[1] - w_f2) <= 1e-13 * w_f2
+    # the two candidates differ by far more than the tolerance, so a series
+    # built from a re-solve of f^{n+1} fails the check above
+    assert abs(w_new - w_f2) > 1e-5 * w_f2
