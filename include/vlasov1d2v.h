@@ -13,9 +13,10 @@
  * axis, one ghost element on each side of every axis, v-line contiguous.
  * E is (NX+2, n) with periodic ghosts; rho is (NX, n).
  *
- * No entry point allocates device memory except vl_geom_create (operator
- * tables, freed by vl_geom_destroy).  Scratch is caller-provided through
- * `workspace` (size from vl_workspace_bytes).  A handle may be used from one
+ * Only the set-up calls allocate device memory (vl_geom_create,
+ * vl_set_background, vl_set_central_flux and the vl_ensemble_* set-up; freed
+ * by the matching destroy).  Scratch is caller-provided through `workspace`
+ * (size from vl_workspace_bytes).  A handle may be used from one
  * host thread at a time; calls on one stream are ordered; no call
  * synchronises the device, so sequences can be captured in a hipGraph.
  */
@@ -129,7 +130,7 @@ int vl_state_elems(vl_geom_t h, size_t *elems);
  * ((ix, iu, iv) inside, elements ordered (ex, eu, ev)) with a pitch of
  * round_up(n^3, 8) values per element, so that a tile of elements is one
  * contiguous run.  It is the resident layout of Stepper.main_loop. */
-/* VL_LAYOUT_ROW_SPLIT (n = 8 or 9 only): elements ordered and padded as in
+/* VL_LAYOUT_ROW_SPLIT (n = 8 or 9 only, states under 2^31 bytes): elements ordered and padded as in
  * the element-major layout; inside an element the first eight nodes of each
  * v-line (ix, iu) form one 16-byte row (rows in (ix, iu) order), and for
  * n = 9 the ninth nodes of all lines follow in (ix, iu) order.  It is the
@@ -191,7 +192,9 @@ int vl_swap_ghosts(vl_geom_t h, void *f, void *stream);
  * elliptic state after a step -- and energy_out[0] (device, may be NULL)
  * = 0.5 sum w E^2 of that field (the value main/timestep.py:102 appends).
  * flags: VL_STEP_RHO_VALID -- the workspace already holds the moment partials
- * of f from the previous vl_rk3_step on the same f (skips one read of f).
+ * of f, left by the previous vl_rk3_step on the same f (or vl_moment_partials),
+ * and nothing has written the workspace or f since (skips one read of f; the
+ * caller vouches for it: a stale workspace gives a wrong first-stage field).
  * VL_STEP_ELEMENT_MAJOR -- f, f1, f2 are in the element-major layout (16-byte
  * aligned).  VL_STEP_ROW_SPLIT -- f, f1, f2 are in the row-split layout
  * (16-byte aligned; BF16/FP16 storage, n = 8 or 9).
@@ -220,6 +223,16 @@ int vl_set_cfl(vl_geom_t h, double courant, double dx, double du, double dv);
 int vl_clock_init(vl_geom_t h, const double *E, double time, void *workspace, void *stream);
 int vl_rk3_step_adaptive(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi, double *energy_out,
                          double *clock_out, void *workspace, int flags, void *stream);
+
+/* The moment partials of f (storage dtype; layout by VL_STEP_ELEMENT_MAJOR /
+ * VL_STEP_ROW_SPLIT in flags, else the reference layout) into the workspace:
+ * the first pass of vl_rk3_step without VL_STEP_RHO_VALID.  A caller that
+ * replays captured steps whose first step was captured with
+ * VL_STEP_RHO_VALID runs it before a replay whenever the workspace or f was
+ * written by anything else since the previous step (the first half of
+ * Distribution.moment_zero_of_arr, main/grid.py:319-321, for the stepper's
+ * own state). */
+int vl_moment_partials(vl_geom_t h, const void *f, void *workspace, int flags, void *stream);
 
 /* Optional timing hook (no counterpart in the reference, whose only timing
  * is a printed wall clock, main/timestep.py:117,139): when `events` is
@@ -295,7 +308,12 @@ int vl_ensemble_moments(vl_ensemble_t e, void *stream);
 /* One SSP-RK3 step of every run.  energy_out (device, may be NULL): count
  * values, run r's field energy of its stage-3 field (as vl_rk3_step).  The
  * first call after vl_ensemble_bind / set_schedule uploads the run table
- * (synchronous): make it outside a graph capture; later calls only launch. */
+ * (synchronous): make it outside a graph capture; later calls only launch.
+ * A graph captured from these launches holds the run table and schedule of
+ * its capture: after vl_ensemble_bind, vl_ensemble_set_schedule,
+ * vl_ensemble_autotune or vl_ensemble_set_adaptive capture it again (earlier
+ * work-item lists stay allocated until vl_ensemble_destroy, so a stale
+ * replay reads valid memory, but it runs the old schedule). */
 int vl_ensemble_rk3_step(vl_ensemble_t e, double *energy_out, void *stream);
 
 /* Autotuner: times the stage-2 launch (reps times each) over x-segment work

@@ -47,10 +47,12 @@ int vl_run_data_create(const char *path, const int64_t shape[6], const float *pd
 
 /* Appends one record to every dataset (each grows by one along axis 0; the
  * five must have the same length).  The buffers have the record shapes the
- * file was created with.
+ * file was created with; *_elems are their value counts, and a count that
+ * differs from the record size is VL_E_ARG (nothing is written).
  * Replaces RunData.save_data (main/data_management.py:37-52). */
-int vl_run_data_append(const char *path, const float *pdf, const float *potential, const float *density,
-                       double time, double energy);
+int vl_run_data_append(const char *path, const float *pdf, int64_t pdf_elems, const float *potential,
+                       int64_t potential_elems, const float *density, int64_t density_elems, double time,
+                       double energy);
 
 /* Rank, current and maximum dims of dataset `name` (maxdims -1 = unlimited). */
 int vl_read_data_dims(const char *path, const char *name, int *rank, int64_t dims[8], int64_t maxdims[8]);

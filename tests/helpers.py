@@ -57,3 +57,46 @@ def significant_ulp_distance(a, b, floor=1e-3):
     big = torch.maximum(a.float().abs(), b.float().abs())
     mask = (big >= floor * float(a.float().abs().max())).cpu().numpy()
     return ulp_distance(a, b)[mask]
+
+
+def reference_lgl(n):
+    """Lobatto nodes and weights as the reference's tables give them
+    (main/basis.py:9-57): the decimal literals of tests/golden/lgl_literals.json
+    for 7..10 nodes, the closed forms (float64 evaluation) below that."""
+    import json
+    import os
+    if n >= 7:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lgl_literals.json")
+        with open(path) as fh:
+            lit = json.load(fh)
+        return (np.array([float(t) for t in lit["nodes"][str(n)]]),
+                np.array([float(t) for t in lit["weights"][str(n)]]))
+    s7 = np.sqrt(7)
+    closed = {
+        2: ([-1, 1], [1, 1]),
+        3: ([-1, 0, 1], [1 / 3, 4 / 3, 1 / 3]),
+        4: ([-1, -np.sqrt(1 / 5), np.sqrt(1 / 5), 1], [1 / 6, 5 / 6, 5 / 6, 1 / 6]),
+        5: ([-1, -np.sqrt(3 / 7), 0, np.sqrt(3 / 7), 1], [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10]),
+        6: ([-1, -np.sqrt(1 / 3 + 2 * s7 / 21), -np.sqrt(1 / 3 - 2 * s7 / 21), np.sqrt(1 / 3 - 2 * s7 / 21),
+             np.sqrt(1 / 3 + 2 * s7 / 21), 1],
+            [1 / 15, (14 - s7) / 30, (14 + s7) / 30, (14 + s7) / 30, (14 - s7) / 30, 1 / 15]),
+    }
+    x, w = closed[n]
+    return np.array(x, dtype=np.float64), np.array(w, dtype=np.float64)
+
+
+def reference_way_up(n):
+    """The volume operator evaluated as main/basis.py:176-188 evaluates it:
+    up[i, j] = w_j sum_s (2s+1)/2 P_s(x_i) P_s'(x_j) with scipy's
+    ``legendre(s)`` polynomial objects (their coefficients come from roots,
+    so odd/even terms are not exactly zero) on the reference's node tables,
+    then entries below 1e-10 cleared.  Not exactly centro-antisymmetric."""
+    import scipy.special as sp
+    x, w = reference_lgl(n)
+    up = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            up[i, j] = w[j] * sum((2 * s + 1) / 2 * sp.legendre(s)(x[i]) * sp.legendre(s).deriv()(x[j])
+                                  for s in range(n))
+    up[np.abs(up) < 1.0e-10] = 0
+    return up

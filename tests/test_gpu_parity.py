@@ -283,3 +283,80 @@ def test_resident_steps_every_order_match_oracle(cuda, n, nx, nu, nv):
     err_e = rel_l2(st.field_energy, st_ref.field_energy)
     print(f"n={n}: rel-L2 f {err_f:.2e}, energy {err_e:.2e}", flush=True)
     assert err_f < 1e-12 and err_e < 1e-12
+
+
+@pytest.mark.parametrize("n", [4, 9, 10])
+def test_rhs_with_the_reference_volume_operator(cuda, n):
+    """A geometry built from the reference's own up table (scipy polynomials
+    on its literal nodes, centro-antisymmetric only to rounding) is accepted
+    and gives the oracle's RHS to 1e-12."""
+    import torch
+    from helpers import reference_way_up
+    from vlasov1d2v_amd import _lib
+    nx, nu, nv = 3, 4, 5
+    case, run = make_pair(n, nx, nu, nv)
+    gr = run.grids
+    b = gr.basis.b1
+    geo = _lib.Geometry(n, nx, nu, nv, reference_way_up(n), b.xi, b.weights, gr.x.arr, gr.u.arr, gr.v.arr,
+                        gr.x.J, gr.u.J, gr.v.J, gr.x.dx, gr.x.length, gr.x.wave_numbers,
+                        gr.x.spectral_transform, gr.x.grid_phases)
+    geo.set_physics(run.flux.acceleration_coefficient, run.elliptic.magnetic_field, run.elliptic.poisson_coefficient)
+    f = random_state((nx + 2, n, nu + 2, n, nv + 2, n), seed=5)
+    E = np.random.default_rng(4).standard_normal((nx + 2, n))
+    case.elliptic.electric_field = E.copy()
+    ref = case.rhs(f)
+    fd = torch.from_numpy(f).to(cuda)
+    Ed = torch.from_numpy(E).to(cuda)
+    out = torch.zeros_like(fd)
+    _lib.check(_lib.load().vl_rhs(geo.handle, _lib.ptr(fd), _lib.ptr(Ed), _lib.ptr(out), _lib.stream_ptr()),
+               "vl_rhs")
+    assert rel_max(interior(out.cpu().numpy()), interior(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_step_after_another_moment_on_the_same_grid(cuda, resident):
+    """Stepping A, taking the moment of another array B on the same grid
+    (which writes the shared workspace's moment partials), then stepping A
+    again gives what two undisturbed steps give: the second step recomputes
+    the moment of A instead of trusting stale partials."""
+    import torch
+    _, run_a = make_pair(5, 4, 5, 6, dt=0.01)
+    _, run_c = make_pair(5, 4, 5, 6, dt=0.01)
+    other = torch.from_numpy(random_state(tuple(run_a.dist.arr.shape), seed=9) + 2.0).to(cuda)
+    if resident:
+        run_a.resident_begin()
+        run_c.resident_begin()
+    run_a.step()
+    run_c.step()
+    run_a.dist.moment_zero_of_arr(other)
+    run_a.step()
+    run_c.step()
+    if resident:
+        run_a.resident_end()
+        run_c.resident_end()
+    torch.cuda.synchronize()
+    assert torch.equal(run_a.dist.arr, run_c.dist.arr)
+
+
+def test_graph_replay_after_another_moment_on_the_same_grid(cuda):
+    """The same for a captured HIP graph, whose first step reuses the
+    partials in the workspace: its replay refreshes them when anything wrote
+    the workspace since the last step."""
+    import torch
+    _, run_a = make_pair(6, 4, 6, 6, dt=0.01)
+    _, run_c = make_pair(6, 4, 6, 6, dt=0.01)
+    other = torch.from_numpy(random_state(tuple(run_a.dist.arr.shape), seed=10) + 2.0).to(cuda)
+    graphs = []
+    for run in (run_a, run_c):
+        run.resident_begin()
+        run.step()
+        graphs.append(run.capture(3))
+    for k in range(3):
+        graphs[0].replay()
+        graphs[1].replay()
+        run_a.dist.moment_zero_of_arr(other)
+    run_a.resident_end()
+    run_c.resident_end()
+    torch.cuda.synchronize()
+    assert torch.equal(run_a.dist.arr, run_c.dist.arr)
+    assert torch.equal(graphs[0].energies, graphs[1].energies)

@@ -144,3 +144,63 @@ def test_ensemble_entry_points_validate_without_a_device():
     assert lib.vl_ensemble_destroy(None) == 0
     with pytest.raises(ValueError):
         Ensemble([], storage="bf16")
+
+
+def _geom_desc(n, up):
+    """A small valid geometry description around the given volume operator."""
+    from vlasov1d2v_amd import _lib
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data
+    d = _lib.VlGeomDesc()
+    d.n, d.nx, d.nu, d.nv, d.n_modes, d.storage = n, 2, 2, 2, 3, 0
+    d.jx = d.ju = d.jv = 1.0
+    d.acc, d.b_field, d.poisson_coef, d.x_dx, d.x_length = -1.0, 0.1, -1.0, 1.0, 2.0
+    d.up = arr(up)
+    d.xi = arr(np.zeros((n, 2)))
+    d.weights = arr(np.ones(n))
+    d.x_nodes = arr(np.zeros(4 * n))
+    d.u_nodes = arr(np.zeros(4 * n))
+    d.v_nodes = arr(np.zeros(4 * n))
+    d.wave_numbers = arr(np.ones(3))
+    d.transform = arr(np.zeros(2 * 2 * n * 3))
+    d.phases = arr(np.zeros(2 * 2 * n * 3))
+    return d, keep
+
+
+@pytest.mark.parametrize("n", list(range(2, 11)))
+def test_geometry_accepts_the_reference_volume_operator(n):
+    """The reference's own up table (scipy polynomials on its literal nodes)
+    is centro-antisymmetric only to rounding; the library accepts it (the
+    validation passes; without a device the call then fails in HIP with a
+    positive code) and rejects a table off by more than 1e-10 of max|up|."""
+    from helpers import reference_way_up
+    from vlasov1d2v_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    up = reference_way_up(n)
+    defect = np.max(np.abs(up + up[::-1, ::-1])) / np.max(np.abs(up))
+    assert defect <= 1e-11
+    h = ctypes.c_void_p()
+    d, keep = _geom_desc(n, up)
+    rc = lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc >= 0, lib.vl_last_error_string()
+    if rc == 0:
+        lib.vl_geom_destroy(h)
+    bad = up.copy()
+    bad[0, 1] += 1e-6 * np.max(np.abs(up))
+    d, keep = _geom_desc(n, bad)
+    assert lib.vl_geom_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"centro-antisymmetric" in lib.vl_last_error_string()
+
+
+def test_reference_volume_operator_is_not_exactly_antisymmetric():
+    """The defect that the tolerance above is for: nonzero at n = 9 and 10."""
+    from helpers import reference_way_up
+    for n in (9, 10):
+        up = reference_way_up(n)
+        assert np.max(np.abs(up + up[::-1, ::-1])) > 0

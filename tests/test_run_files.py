@@ -159,7 +159,21 @@ def test_errors_are_reported(tmp_path):
     w = dm.RunData(str(tmp_path) + os.sep, "run", SHAPE, None, None)
     with pytest.raises(ValueError, match="state shape"):
         w.save_data(np.zeros((2, 2)), _Field(np.zeros((3, 3))), np.zeros((3, 3)), 0.0, 0.0)
+    # potential / density records must have the file's (NX, n) shape
+    with pytest.raises(ValueError, match="records are"):
+        w.save_data(np.zeros(SHAPE), _Field(np.zeros((3, 2))), np.zeros((3, 2)), 0.0, 0.0)
+    with pytest.raises(ValueError, match="must be"):
+        w.save_data(np.zeros(SHAPE), _Field(np.zeros((3, 3))), np.zeros((3, 2)), 0.0, 0.0)
     w.close()
+    # the C side checks the buffer sizes itself and leaves the file unchanged
+    path = str(tmp_path / "run.hdf5").encode()
+    before = dm.ReadData(str(tmp_path) + os.sep, "run")._records("pdf")
+    pdf = np.zeros(SHAPE, dtype=np.float32)
+    small = np.zeros(4, dtype=np.float32)
+    assert lib.vl_run_data_append(path, pdf.ctypes.data, pdf.size, small.ctypes.data, small.size,
+                                  small.ctypes.data, small.size, 0.0, 0.0) == -1
+    assert b"buffer 4" in lib.vl_io_last_error_string()
+    assert dm.ReadData(str(tmp_path) + os.sep, "run")._records("pdf") == before
 
 
 def test_read_info_parses_the_reference_lines(tmp_path):

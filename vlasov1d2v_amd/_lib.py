@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvlasov1d2v.so")
 
 VL_F64, VL_F32, VL_BF16, VL_F16 = 0, 1, 2, 3
+VL_OK, VL_E_ARG, VL_E_NULL, VL_E_ALIGN, VL_E_HANDLE, VL_E_WORKSPACE, VL_E_NOTFINITE = 0, -1, -2, -3, -4, -5, -6
 VL_STEP_RHO_VALID = 1
 
 # Every symbol the header declares (checked by the CPU test suite).
@@ -25,6 +26,7 @@ EXPORTS = (
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
+    "vl_moment_partials",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -104,6 +106,7 @@ def load(path=LIB_PATH):
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
+        "vl_moment_partials": [vp, vp, vp, i32, vp],
         "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
         "vl_ensemble_destroy": [vp],
         "vl_ensemble_set_adaptive": [vp, i32, i32],
@@ -126,7 +129,26 @@ def load(path=LIB_PATH):
     return lib
 
 
+# Entry points that write a workspace's moment partials or a state array.
+# A stepper passes VL_STEP_RHO_VALID only while none of them ran since its
+# own previous step (see ``generation``).
+WRITERS = frozenset({
+    "vl_moment0", "vl_moment_partials", "vl_rk3_step", "vl_rk3_step_adaptive",
+    "vl_convert_layout",
+    "vl_import_f32", "vl_init_eigen", "vl_ensemble_rk3_step", "vl_ensemble_moments", "vl_ensemble_autotune",
+})
+_generation = 0
+
+
+def generation():
+    """Count of the calls of WRITERS entry points made so far (through check)."""
+    return _generation
+
+
 def check(rc, what):
+    global _generation
+    if what.split("(")[0] in WRITERS:
+        _generation += 1
     if rc != 0:
         msg = load().vl_last_error_string().decode(errors="replace")
         raise VlasovError(f"{what} failed (code {rc}): {msg}")

@@ -125,7 +125,35 @@ done:
 }
 
 /* grows dataset `name` by one record and writes `data` into it */
-static int append_record(hid_t file, const char *name, hid_t mtype, const void *data, hsize_t expect_len) {
+/* VL_OK_ when a record of dataset `name` holds `elems` values */
+static int check_record(hid_t file, const char *name, int64_t elems) {
+    int rc = VL_E_IO;
+    hsize_t dims[8];
+    hid_t ds = H5Dopen2(file, name, H5P_DEFAULT), sp = -1;
+    if (ds < 0) { fail(VL_E_IO, "no dataset '%s'", name); goto done; }
+    sp = H5Dget_space(ds);
+    const int rank = sp < 0 ? -1 : H5Sget_simple_extent_ndims(sp);
+    if (rank < 1 || rank > 8 || H5Sget_simple_extent_dims(sp, dims, NULL) < 0) {
+        fail(VL_E_IO, "bad dataspace of '%s'", name);
+        goto done;
+    }
+    int64_t rec = 1;
+    for (int i = 1; i < rank; ++i) rec *= (int64_t)dims[i];
+    if (rec != elems) {
+        rc = fail(VL_E_ARG_, "a record of '%s' holds %lld values, the buffer %lld", name, (long long)rec,
+                  (long long)elems);
+        goto done;
+    }
+    rc = VL_OK_;
+done:
+    if (sp >= 0) H5Sclose(sp);
+    if (ds >= 0) H5Dclose(ds);
+    return rc;
+}
+
+/* elems: values in the caller's buffer, checked against the record shape */
+static int append_record(hid_t file, const char *name, hid_t mtype, const void *data, int64_t elems,
+                         hsize_t expect_len) {
     int rc = VL_E_IO;
     hid_t ds = -1, fspace = -1, mspace = -1;
     hsize_t dims[8], start[8], count[8];
@@ -141,6 +169,15 @@ static int append_record(hid_t file, const char *name, hid_t mtype, const void *
         rc = fail(VL_E_ARG_, "dataset '%s' has %llu records, 'pdf' has %llu", name, (unsigned long long)dims[0],
                   (unsigned long long)expect_len);
         goto done;
+    }
+    {
+        int64_t rec = 1;
+        for (int i = 1; i < rank; ++i) rec *= (int64_t)dims[i];
+        if (rec != elems) {
+            rc = fail(VL_E_ARG_, "a record of '%s' holds %lld values, the buffer %lld", name, (long long)rec,
+                      (long long)elems);
+            goto done;
+        }
     }
     H5Sclose(fspace);
     fspace = -1;
@@ -186,8 +223,9 @@ static int record_count(hid_t file, const char *name, hsize_t *len) {
     return rc;
 }
 
-int vl_run_data_append(const char *path, const float *pdf, const float *potential, const float *density,
-                       double time, double energy) {
+int vl_run_data_append(const char *path, const float *pdf, int64_t pdf_elems, const float *potential,
+                       int64_t potential_elems, const float *density, int64_t density_elems, double time,
+                       double energy) {
     if (!path || !pdf || !potential || !density) return fail(VL_E_NULL_, "vl_run_data_append: null pointer");
     pthread_mutex_lock(&g_mu);
     quiet_errors();
@@ -199,11 +237,15 @@ int vl_run_data_append(const char *path, const float *pdf, const float *potentia
     }
     hsize_t len = 0;
     int rc = record_count(file, "pdf", &len);
-    if (!rc) rc = append_record(file, "pdf", H5T_NATIVE_FLOAT, pdf, len);
-    if (!rc) rc = append_record(file, "potential", H5T_NATIVE_FLOAT, potential, len);
-    if (!rc) rc = append_record(file, "density", H5T_NATIVE_FLOAT, density, len);
-    if (!rc) rc = append_record(file, "time", H5T_NATIVE_DOUBLE, &time, len);
-    if (!rc) rc = append_record(file, "energy", H5T_NATIVE_DOUBLE, &energy, len);
+    /* every size first, so that a mismatch leaves the file unchanged */
+    if (!rc) rc = check_record(file, "pdf", pdf_elems);
+    if (!rc) rc = check_record(file, "potential", potential_elems);
+    if (!rc) rc = check_record(file, "density", density_elems);
+    if (!rc) rc = append_record(file, "pdf", H5T_NATIVE_FLOAT, pdf, pdf_elems, len);
+    if (!rc) rc = append_record(file, "potential", H5T_NATIVE_FLOAT, potential, potential_elems, len);
+    if (!rc) rc = append_record(file, "density", H5T_NATIVE_FLOAT, density, density_elems, len);
+    if (!rc) rc = append_record(file, "time", H5T_NATIVE_DOUBLE, &time, 1, len);
+    if (!rc) rc = append_record(file, "energy", H5T_NATIVE_DOUBLE, &energy, 1, len);
     if (H5Fclose(file) < 0 && !rc) rc = fail(VL_E_IO, "closing '%s' failed", path);
     pthread_mutex_unlock(&g_mu);
     return rc;

@@ -66,6 +66,9 @@ struct vl_geom {
 
 namespace {
 
+// relative tolerance of the centro-antisymmetry check of the volume operator
+constexpr double kUpSymTol = 1e-10;
+
 thread_local char g_err[512] = "";
 
 int fail(int code, const char *fmt, ...) {
@@ -464,6 +467,14 @@ std::vector<unsigned> build_mxfrag(const OpTables &tab, int n) {
     return out;
 }
 
+// The row-split kernels address a whole state array through one buffer
+// resource with 32-bit byte offsets (and an out-of-range offset of
+// 0x7fffffff that must fall outside it): states of 2^31 bytes or more are
+// refused.
+bool row_split_fits(const vl_geom *h) {
+    return (int64_t)(h->nx + 2) * h->sxe_elem * 2 < 2147483647LL;
+}
+
 int check_handle(const vl_geom *h) {
     if (!h) return fail(VL_E_HANDLE, "null handle");
     return 0;
@@ -579,16 +590,31 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         h->tab.xi0[i] = d->xi[2 * i];
         h->tab.xi1[i] = d->xi[2 * i + 1];
     }
-    // even-odd split of up (exact: the Lobatto volume operator is
-    // centro-antisymmetric; checked here)
+    // even-odd split of up.  The Lobatto volume operator is
+    // centro-antisymmetric, up[n-1-i][n-1-j] = -up[i][j]; a table built in
+    // floating point (the reference evaluates polynomials whose odd/even
+    // coefficients are not exactly zero, main/basis.py:176-188) misses that by
+    // rounding (4e-13 of max|up| at n = 9).  Tables within kUpSymTol of it are
+    // accepted and made exactly antisymmetric (the mean of each entry pair),
+    // which every kernel then uses; larger defects are an error.
     {
         const int H = n / 2, M = n / 2;
+        double amax = 0.0;
+        for (int i = 0; i < n * n; ++i) amax = std::max(amax, std::fabs(h->tab.up[i]));
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double a = h->tab.up[i * n + j], b = h->tab.up[(n - 1 - i) * n + (n - 1 - j)];
+                if (!std::isfinite(a) || std::fabs(a + b) > kUpSymTol * amax) {
+                    delete h;
+                    return fail(VL_E_ARG, "volume operator is not centro-antisymmetric at (%d,%d): %.3e",
+                                i, j, std::fabs(a + b));
+                }
+            }
+        double sym[kMaxN * kMaxN];
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j)
-                if (h->tab.up[(n - 1 - i) * n + (n - 1 - j)] != -h->tab.up[i * n + j]) {
-                    delete h;
-                    return fail(VL_E_ARG, "volume operator is not centro-antisymmetric at (%d,%d)", i, j);
-                }
+                sym[i * n + j] = 0.5 * (h->tab.up[i * n + j] - h->tab.up[(n - 1 - i) * n + (n - 1 - j)]);
+        for (int i = 0; i < n * n; ++i) h->tab.up[i] = sym[i];
         for (int i = 0; i < H; ++i) {
             for (int j = 0; j < H; ++j) {
                 h->tab.eoP[i * H + j] = 0.5 * (h->tab.up[i * n + j] + h->tab.up[(n - 1 - i) * n + j]);
@@ -855,6 +881,9 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
         return fail(VL_E_ALIGN, "%s: element-wise arrays must be 16-byte aligned", what);
     if (LAY == kRowSplitLayout && ((h->storage != VL_BF16 && h->storage != VL_F16) || (h->n != 8 && h->n != 9)))
         return fail(VL_E_ARG, "%s: the row-split layout needs BF16/FP16 storage and n = 8 or 9", what);
+    if (LAY == kRowSplitLayout && !row_split_fits(h))
+        return fail(VL_E_ARG, "%s: the row-split kernels address the state with 32-bit offsets; this state "
+                              "has 2^31 bytes or more (use the element-major layout)", what);
     hipStream_t st = (hipStream_t)stream;
     if (int r = swap_ghosts_layout(h, f, LAY, stream)) return r;
     StageArgs s = make_stage_args(h, LAY);
@@ -943,6 +972,25 @@ int vl_rk3_step_adaptive(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E,
                     "vl_rk3_step_adaptive");
 }
 
+int vl_moment_partials(vl_geom_t h, const void *f, void *ws, int flags, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!f || !ws) return fail(VL_E_NULL, "vl_moment_partials: null pointer");
+    if ((flags & VL_STEP_ELEMENT_MAJOR) && (flags & VL_STEP_ROW_SPLIT))
+        return fail(VL_E_ARG, "vl_moment_partials: VL_STEP_ELEMENT_MAJOR and VL_STEP_ROW_SPLIT are exclusive");
+    const int LAY = (flags & VL_STEP_ROW_SPLIT) ? kRowSplitLayout
+                    : (flags & VL_STEP_ELEMENT_MAJOR) ? kElemLayout : kRefLayout;
+    const size_t eb = storage_bytes(h->storage);
+    if (!aligned(f, LAY != kRefLayout ? 16 : eb) || !aligned(ws, 256))
+        return fail(VL_E_ALIGN, "vl_moment_partials: misaligned pointer");
+    if (LAY == kRowSplitLayout && ((h->storage != VL_BF16 && h->storage != VL_F16) || (h->n != 8 && h->n != 9) ||
+                                   !row_split_fits(h)))
+        return fail(VL_E_ARG, "vl_moment_partials: no row-split layout for this geometry");
+    StageArgs s = make_stage_args(h, LAY);
+    s.partials = ws_partials(h, ws);
+    VL_HIP(launch_moment(h, h->storage, LAY, f, s, (hipStream_t)stream));
+    return VL_OK;
+}
+
 int vl_state_elems_layout(vl_geom_t h, int layout, size_t *elems) {
     if (int r = check_handle(h)) return r;
     if (!elems) return fail(VL_E_NULL, "null elems");
@@ -965,6 +1013,8 @@ int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, i
     const int lay = to_elem ? dst_layout : src_layout;
     if (lay == VL_LAYOUT_ROW_SPLIT && h->n != 8 && h->n != 9)
         return fail(VL_E_ARG, "the row-split layout needs n = 8 or 9 (n = %d)", h->n);
+    if (lay == VL_LAYOUT_ROW_SPLIT && !row_split_fits(h))
+        return fail(VL_E_ARG, "vl_convert_layout: a row-split state of 2^31 bytes or more is not supported");
     const size_t eb = storage_bytes(h->storage);
     if (!aligned(src, eb) || !aligned(dst, eb)) return fail(VL_E_ALIGN, "vl_convert_layout: misaligned");
     const int64_t total = (int64_t)h->state_elems;
@@ -1047,7 +1097,7 @@ int vl_resident_layout(vl_geom_t h, int *layout) {
     if (int r = check_handle(h)) return r;
     if (!layout) return fail(VL_E_NULL, "vl_resident_layout: null layout");
     const bool rs = (h->storage == VL_BF16 || h->storage == VL_F16) && (h->n == 8 || h->n == 9) && h->mfma &&
-                    h->row_split;
+                    h->row_split && row_split_fits(h);
     *layout = rs ? VL_LAYOUT_ROW_SPLIT : VL_LAYOUT_ELEMENT_MAJOR;
     return VL_OK;
 }
@@ -1256,6 +1306,7 @@ struct vl_ensemble {
     std::vector<int2> items;
     int2 *d_items;
     size_t items_cap;
+    std::vector<int2 *> retired;   // earlier item lists (freed by vl_ensemble_destroy)
     // field kernels: workgroup maps
     EnsBlock *d_map_swap, *d_map_red, *d_map_fwd, *d_map_inv;
     int nb_swap, nb_red, nb_fwd, nb_inv;
@@ -1312,7 +1363,9 @@ int ens_upload(vl_ensemble *e) {
     if (!e->dirty) return VL_OK;
     VL_HIP(hipMemcpy(e->d_runs, e->runs.data(), e->runs.size() * sizeof(EnsRun), hipMemcpyHostToDevice));
     if (e->items.size() > e->items_cap) {
-        if (e->d_items) VL_HIP(hipFree(e->d_items));
+        // a graph captured earlier may still launch over the old list: keep
+        // it allocated until the ensemble is destroyed
+        if (e->d_items) e->retired.push_back(e->d_items);
         e->d_items = nullptr;
         VL_HIP(hipMalloc((void **)&e->d_items, e->items.size() * sizeof(int2)));
         e->items_cap = e->items.size();
@@ -1473,6 +1526,7 @@ int vl_ensemble_destroy(vl_ensemble_t e) {
     void *bufs[] = {e->d_runs, e->d_items, e->d_map_swap, e->d_map_red, e->d_map_fwd, e->d_map_inv};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (int2 *b : e->retired) (void)hipFree(b);
     delete e;
     return VL_OK;
 }

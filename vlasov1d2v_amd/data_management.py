@@ -47,7 +47,7 @@ def load_io(path=IO_LIB_PATH):
     p64 = ctypes.POINTER(ctypes.c_int64)
     sig = {
         "vl_run_data_create": [cp, p64, vp, i64, i64, vp, vp],
-        "vl_run_data_append": [cp, vp, vp, vp, f64, f64],
+        "vl_run_data_append": [cp, vp, i64, vp, i64, vp, i64, f64, f64],
         "vl_read_data_dims": [cp, cp, ctypes.POINTER(i32), p64, p64],
         "vl_read_data": [cp, cp, i64, i64, i32, vp],
     }
@@ -65,6 +65,20 @@ def _check_io(rc, what):
     if rc != 0:
         msg = load_io().vl_io_last_error_string().decode(errors="replace")
         raise _lib.VlasovError(f"{what} failed ({rc}): {msg}")
+
+
+def _dataset_dims(path, name):
+    """(dims, maxdims) of dataset `name` of the run file at `path`."""
+    rank = ctypes.c_int()
+    d = (ctypes.c_int64 * 8)()
+    m = (ctypes.c_int64 * 8)()
+    _check_io(load_io().vl_read_data_dims(path.encode(), name.encode(), ctypes.byref(rank), d, m),
+              f"vl_read_data_dims({name})")
+    return tuple(d[:rank.value]), tuple(m[:rank.value])
+
+
+def _numel(a):
+    return int(a.numel()) if hasattr(a, "numel") else int(a.size)
 
 
 def _host_f32(a):
@@ -175,9 +189,15 @@ class RunData:
     def save_data(self, distribution, elliptic, density, time, field_energy):
         """Appends a record (main/data_management.py:37-52).  Returns once the
         state is staged; the file write runs on the writer thread."""
-        pdf, ev, slot = self._pdf_record(distribution)
         pot = _host_f32(elliptic.potential)
         den = _host_f32(density)
+        # the record shapes of the file (potential and density share theirs)
+        if pot.ndim != 2 or den.shape != pot.shape:
+            raise ValueError(f"potential {pot.shape} and density {den.shape} must be (NX, n)")
+        want = tuple(_dataset_dims(self.write_filename, "potential")[0][1:])
+        if pot.shape != want:
+            raise ValueError(f"potential/density records are {want}, got {pot.shape}")
+        pdf, ev, slot = self._pdf_record(distribution)
         t = float(time)
         e = float(np.asarray(field_energy.cpu() if hasattr(field_energy, "cpu") else field_energy))
         path = self.write_filename.encode()
@@ -186,7 +206,8 @@ class RunData:
         def write():
             if ev is not None:
                 ev.synchronize()
-            _check_io(lib.vl_run_data_append(path, self._ptr(pdf), self._ptr(pot), self._ptr(den), t, e),
+            _check_io(lib.vl_run_data_append(path, self._ptr(pdf), _numel(pdf), self._ptr(pot), pot.size,
+                                             self._ptr(den), den.size, t, e),
                       "vl_run_data_append")
 
         fut = self._pool.submit(write)
@@ -228,12 +249,7 @@ class ReadData:
 
     def dims(self, name):
         """(dims, maxdims) of a dataset; maxdims -1 = unlimited."""
-        rank = ctypes.c_int()
-        d = (ctypes.c_int64 * 8)()
-        m = (ctypes.c_int64 * 8)()
-        _check_io(load_io().vl_read_data_dims(self.write_filename.encode(), name.encode(), ctypes.byref(rank), d, m),
-                  f"vl_read_data_dims({name})")
-        return tuple(d[:rank.value]), tuple(m[:rank.value])
+        return _dataset_dims(self.write_filename, name)
 
     def _read(self, name, start, count, as_double):
         dims, _ = self.dims(name)

@@ -134,6 +134,7 @@ class Ensemble:
         _lib.check(lib.vl_ensemble_moments(h, _lib.stream_ptr()), "vl_ensemble_moments")
         self.steps_done = 0
         self.tuning = None
+        self.schedule_generation = 0     # bumped when the schedule changes (captured graphs go stale)
 
     @property
     def dof(self):
@@ -149,6 +150,7 @@ class Ensemble:
     def set_schedule(self, work_per_item, grid=0, order=0):
         _lib.check(_lib.load().vl_ensemble_set_schedule(self.handle, int(work_per_item), int(grid), int(order)),
                    "vl_ensemble_set_schedule")
+        self.schedule_generation += 1
 
     def autotune(self, reps=2):
         """Time the candidate schedules of the stage kernel and keep the fastest."""
@@ -156,6 +158,7 @@ class Ensemble:
         rows = ctypes.c_int()
         _lib.check(_lib.load().vl_ensemble_autotune(self.handle, int(reps), table, 64, ctypes.byref(rows),
                                                     _lib.stream_ptr()), "vl_ensemble_autotune")
+        self.schedule_generation += 1
         t = np.frombuffer(table, dtype=np.float32)[:4 * rows.value].reshape(-1, 4)
         self.tuning = [{"work_per_item": int(a), "grid": int(b), "order": int(c), "ms": float(d)}
                        for a, b, c, d in t]
@@ -218,7 +221,9 @@ class Ensemble:
 
 class EnsembleGraph:
     """``steps`` ensemble steps captured once; ``replay()`` advances every run
-    by ``steps`` steps and leaves the energies in ``energies`` (steps, runs)."""
+    by ``steps`` steps and leaves the energies in ``energies`` (steps, runs).
+    The graph holds the schedule of its capture: ``replay()`` raises once
+    ``set_schedule`` or ``autotune`` has changed it (capture again)."""
 
     def __init__(self, ens, steps):
         import torch
@@ -227,6 +232,7 @@ class EnsembleGraph:
         dev = ens.device
         self.steps = int(steps)
         self.ens = ens
+        self.schedule_generation = ens.schedule_generation
         self.energies = torch.zeros((self.steps, len(ens.runs)), dtype=torch.float64, device=dev)
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(torch.cuda.current_stream(dev))
@@ -245,6 +251,8 @@ class EnsembleGraph:
         torch.cuda.current_stream(dev).wait_stream(side)
 
     def replay(self):
+        if self.ens.schedule_generation != self.schedule_generation:
+            raise _lib.VlasovError("the ensemble schedule changed after this graph was captured: capture again")
         self.graph.replay()
         self.ens.steps_done += self.steps
 

@@ -158,11 +158,10 @@ class Stepper:
         if self.fixed_dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
 
-    def _fused_step(self, distribution, elliptic, grids, dg_flux, energy_out=None, clock_out=None,
-                    adaptive=False):
-        """vl_rk3_step with self.dt, or (adaptive) vl_rk3_step_adaptive with
-        the workspace clock; clock_out: 2-element GPU tensor or None."""
-        import torch
+    def _step_state(self, distribution, grids, dg_flux, elliptic):
+        """(geometry, state array the step works on, its stage buffers, layout
+        flags): the resident buffers while resident_begin is active for this
+        distribution, else distribution.arr in the reference layout."""
         arr = distribution.arr
         _check_state(arr, grids.shape, "nonlinear_ssp_rk", allow_reduced=True)
         g = dg_flux._geom(grids, elliptic, STORAGE_CODES[storage_of(arr)])
@@ -170,8 +169,26 @@ class Stepper:
                             and self._res["dist"] is distribution) else None
         if res is not None:
             arr, f1, f2 = res["bufs"]
+            lay = (_lib.VL_STEP_ROW_SPLIT if res["layout"] == _lib.VL_LAYOUT_ROW_SPLIT
+                   else _lib.VL_STEP_ELEMENT_MAJOR)
         else:
             f1, f2 = self._stage_buffers(arr)
+            lay = 0
+        return g, arr, f1, f2, lay
+
+    @staticmethod
+    def _token(arr, ws):
+        """What must be unchanged since a step for its moment partials, left in
+        the workspace, to describe ``arr``: the array, its torch version, the
+        workspace and the count of library calls that write partials or states."""
+        return (arr.data_ptr(), arr._version, ws.data_ptr(), _lib.generation())
+
+    def _fused_step(self, distribution, elliptic, grids, dg_flux, energy_out=None, clock_out=None,
+                    adaptive=False):
+        """vl_rk3_step with self.dt, or (adaptive) vl_rk3_step_adaptive with
+        the workspace clock; clock_out: 2-element GPU tensor or None."""
+        import torch
+        g, arr, f1, f2, lay = self._step_state(distribution, grids, dg_flux, elliptic)
         dev = arr.device
         if elliptic.electric_field is None or elliptic.electric_field.device != dev:
             elliptic.electric_field = torch.zeros((grids.x.res_ghosts, grids.x.order), dtype=torch.float64,
@@ -179,11 +196,7 @@ class Stepper:
         if elliptic.potential is None or elliptic.potential.device != dev:
             elliptic.potential = torch.zeros((grids.x.res, grids.x.order), dtype=torch.float64, device=dev)
         ws = g.workspace(dev)
-        token = (arr.data_ptr(), arr._version, ws.data_ptr())
-        flags = _lib.VL_STEP_RHO_VALID if token == self._rho_token else 0
-        if res is not None:
-            flags |= (_lib.VL_STEP_ROW_SPLIT if res["layout"] == _lib.VL_LAYOUT_ROW_SPLIT
-                      else _lib.VL_STEP_ELEMENT_MAJOR)
+        flags = lay | (_lib.VL_STEP_RHO_VALID if self._token(arr, ws) == self._rho_token else 0)
         if adaptive:
             _lib.check(_lib.load().vl_rk3_step_adaptive(
                 g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
@@ -194,7 +207,7 @@ class Stepper:
                 g.handle, _lib.ptr(arr), _lib.ptr(f1), _lib.ptr(f2), _lib.ptr(elliptic.electric_field),
                 _lib.ptr(elliptic.potential), float(self.dt), _lib.ptr(energy_out), _lib.ptr(ws), flags,
                 _lib.stream_ptr()), "vl_rk3_step")
-        self._rho_token = (arr.data_ptr(), arr._version, ws.data_ptr())
+        self._rho_token = self._token(arr, ws)
 
     def ssp_rk_update(self, distribution, basis, grids, elliptic, dg_flux, refs=None):
         """The linear SSP-RK update (main/timestep.py:189-229) on the GPU, in
@@ -403,8 +416,11 @@ class StepGraph:
     allocation, no memset), so it is captured as is on a side stream of
     default priority.  The first captured step reuses the moment partials the
     previous step left in the workspace, so the state must have been stepped
-    once (eagerly) before capture.  ``replay()`` advances the distribution by
-    ``steps`` steps and leaves the energies of those steps in ``energies``.
+    once (eagerly) before capture; when a library call or a torch write has
+    touched the workspace or the state since the last step, ``replay()``
+    first recomputes those partials (``vl_moment_partials``).  ``replay()``
+    advances the distribution by ``steps`` steps and leaves the energies of
+    those steps in ``energies``.
     """
 
     def __init__(self, stepper, distribution, basis, elliptic, grids, dg_flux, steps, refs=None, adaptive=False):
@@ -439,6 +455,16 @@ class StepGraph:
             if gc_was_enabled:
                 gc.enable()
         torch.cuda.current_stream(dev).wait_stream(side)
+        self.stepper = stepper
+        self._geom, self._arr, _, _, self._lay = stepper._step_state(distribution, grids, dg_flux, elliptic)
+        self._ws = self._geom.workspace(dev)
+        # the partials in the workspace are still those of the last eager step
+        self._mark = Stepper._token(self._arr, self._ws)
 
     def replay(self):
+        if Stepper._token(self._arr, self._ws) != self._mark:
+            _lib.check(_lib.load().vl_moment_partials(self._geom.handle, _lib.ptr(self._arr), _lib.ptr(self._ws),
+                                                      self._lay, _lib.stream_ptr()), "vl_moment_partials")
         self.graph.replay()
+        self._mark = Stepper._token(self._arr, self._ws)
+        self.stepper._rho_token = self._mark
