@@ -16,7 +16,8 @@
  * Only the set-up calls allocate device memory (vl_geom_create,
  * vl_set_background, vl_set_central_flux and the vl_ensemble_* set-up; freed
  * by the matching destroy).  Scratch is caller-provided through `workspace`
- * (size from vl_workspace_bytes).  A handle may be used from one
+ * (size from vl_workspace_bytes), or for vl_rk3_step_noalloc a library-owned
+ * array.  A handle may be used from one
  * host thread at a time; calls on one stream are ordered; no call
  * synchronises the device, so sequences can be captured in a hipGraph.
  */
@@ -223,6 +224,25 @@ int vl_set_cfl(vl_geom_t h, double courant, double dx, double du, double dv);
 int vl_clock_init(vl_geom_t h, const double *E, double time, void *workspace, void *stream);
 int vl_rk3_step_adaptive(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi, double *energy_out,
                          double *clock_out, void *workspace, int flags, void *stream);
+
+/* Allocation-free SSP-RK3 step.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  vl_rk3_step with the
+ * workspace taken from a library-owned device array of
+ * VL_NOALLOC_SCRATCH_BYTES per GPU (resolved when the first geometry is
+ * created on that GPU, held until the process ends) instead of the caller's;
+ * the caller passes f, f1, f2, E, phi and energy_out as for vl_rk3_step.  It
+ * calls no hipMalloc / hipFree.  Geometries whose vl_workspace_bytes exceeds
+ * the array are refused with VL_E_WORKSPACE (the 116.6M-DOF n = 9,
+ * 64 x 50 x 50 grid needs 24 MB).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on a GPU (two streams, two threads) race, and
+ * VL_STEP_RHO_VALID is valid only when the previous call on the GPU was a
+ * step of the same f.  The reduction ticket is cleared by a small kernel at
+ * every call; the stochastic-rounding seed of the 16-bit forms carries over
+ * between calls. */
+#define VL_NOALLOC_SCRATCH_BYTES (32u << 20)
+int vl_rk3_step_noalloc(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi, double dt,
+                        double *energy_out, int flags, void *stream);
 
 /* The moment partials of f (storage dtype; layout by VL_STEP_ELEMENT_MAJOR /
  * VL_STEP_ROW_SPLIT in flags, else the reference layout) into the workspace:

@@ -26,7 +26,7 @@ EXPORTS = (
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
-    "vl_moment_partials",
+    "vl_moment_partials", "vl_rk3_step_noalloc",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -107,6 +107,7 @@ def load(path=LIB_PATH):
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
         "vl_moment_partials": [vp, vp, vp, i32, vp],
+        "vl_rk3_step_noalloc": [vp, vp, vp, vp, vp, vp, f64, vp, i32, vp],
         "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
         "vl_ensemble_destroy": [vp],
         "vl_ensemble_set_adaptive": [vp, i32, i32],
@@ -133,7 +134,7 @@ def load(path=LIB_PATH):
 # A stepper passes VL_STEP_RHO_VALID only while none of them ran since its
 # own previous step (see ``generation``).
 WRITERS = frozenset({
-    "vl_moment0", "vl_moment_partials", "vl_rk3_step", "vl_rk3_step_adaptive",
+    "vl_moment0", "vl_moment_partials", "vl_rk3_step", "vl_rk3_step_adaptive", "vl_rk3_step_noalloc",
     "vl_convert_layout",
     "vl_import_f32", "vl_init_eigen", "vl_ensemble_rk3_step", "vl_ensemble_moments", "vl_ensemble_autotune",
 })

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -68,6 +69,34 @@ namespace {
 
 // relative tolerance of the centro-antisymmetry check of the volume operator
 constexpr double kUpSymTol = 1e-10;
+
+// Workspace of vl_rk3_step_noalloc: one library-owned array per GPU (each
+// device holds its own copy until the process ends).  In host code `scratch`
+// names a host-side stand-in, not a device address: kernels and copies use
+// the per-device address that hipGetSymbolAddress returns (g_scratch).
+constexpr size_t kScratchBytes = (size_t)VL_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// Resolves the device address of `scratch` for `device` (current), once.
+hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = p;
+    return hipSuccess;
+}
+
+void *scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
 
 thread_local char g_err[512] = "";
 
@@ -730,6 +759,8 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         e = alloc_copy(&h->d_T, reinterpret_cast<const double2 *>(d->transform), (size_t)K * h->M);
     if (e == hipSuccess)
         e = alloc_copy(&h->d_phases, reinterpret_cast<const double2 *>(d->phases), (size_t)K * h->M);
+    // set-up of the allocation-free entry point on this (current) device
+    if (e == hipSuccess) e = resolve_scratch(h->device);
     if (e != hipSuccess) {
         vl_geom_destroy(h);
         return hip_fail(e, "vl_geom_create");
@@ -970,6 +1001,23 @@ int vl_rk3_step_adaptive(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E,
     if (clock_out && !aligned(clock_out, 8)) return fail(VL_E_ALIGN, "vl_rk3_step_adaptive: misaligned clock_out");
     return rk3_step(h, fv, f1v, f2v, E, phi, 0.0, true, energy_out, clock_out, ws, flags, stream,
                     "vl_rk3_step_adaptive");
+}
+
+int vl_rk3_step_noalloc(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi, double dt,
+                        double *energy_out, int flags, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (h->ws_bytes > kScratchBytes)
+        return fail(VL_E_WORKSPACE, "vl_rk3_step_noalloc: this geometry needs a %zu-byte workspace, above the "
+                                    "%zu bytes of the library's scratch array (use vl_rk3_step)",
+                    h->ws_bytes, kScratchBytes);
+    void *ws = scratch_of(h->device);
+    if (!ws) return fail(VL_E_ARG, "vl_rk3_step_noalloc: no scratch array on device %d", h->device);
+    // the array keeps the previous call's contents (maybe another geometry's
+    // layout): the reduction ticket must start at zero
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, (hipStream_t)stream, ws_ticket(h, ws), 1);
+    VL_HIP(hipGetLastError());
+    return rk3_step(h, fv, f1v, f2v, E, phi, dt, false, energy_out, nullptr, ws, flags, stream,
+                    "vl_rk3_step_noalloc");
 }
 
 int vl_moment_partials(vl_geom_t h, const void *f, void *ws, int flags, void *stream) {
