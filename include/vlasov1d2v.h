@@ -244,6 +244,18 @@ int vl_rk3_step_adaptive(vl_geom_t h, void *f, void *f1, void *f2, double *E, do
 int vl_rk3_step_noalloc(vl_geom_t h, void *f, void *f1, void *f2, double *E, double *phi, double dt,
                         double *energy_out, int flags, void *stream);
 
+/* Run-time failure detection (the reference has none: its NaN check is
+ * commented out, main/timestep.py:118-121).  Every step's stage-3 field solve
+ * (vl_rk3_step, vl_rk3_step_adaptive, vl_ensemble_rk3_step per run) reduces
+ * the field energy and max|acc E| of its field and sets VL_STATUS_NONFINITE in
+ * the workspace's status word when either is not finite -- the state has
+ * overflowed (an unstable time step) or holds NaN.  The bit stays set until
+ * cleared.  vl_read_status copies the word to *status (host), clears it when
+ * `clear` is nonzero, and synchronises `stream`: call it between graph
+ * replays, not inside a capture. */
+enum { VL_STATUS_NONFINITE = 1 };
+int vl_read_status(vl_geom_t h, void *workspace, int clear, unsigned *status, void *stream);
+
 /* The moment partials of f (storage dtype; layout by VL_STEP_ELEMENT_MAJOR /
  * VL_STEP_ROW_SPLIT in flags, else the reference layout) into the workspace:
  * the first pass of vl_rk3_step without VL_STEP_RHO_VALID.  A caller that

@@ -204,3 +204,24 @@ def test_reference_volume_operator_is_not_exactly_antisymmetric():
     for n in (9, 10):
         up = reference_way_up(n)
         assert np.max(np.abs(up + up[::-1, ::-1])) > 0
+
+
+def test_status_word_raises_on_a_non_finite_field():
+    """Host side of the failure detection: a workspace status word with the
+    non-finite bit set (as a stage-3 field solve writes it) raises, a clear
+    word does not -- checked on a stand-in geometry whose status is injected."""
+    from vlasov1d2v_amd import _lib
+    _lib.raise_for_status(0, "step")
+    with pytest.raises(_lib.VlasovError, match="not finite"):
+        _lib.raise_for_status(_lib.VL_STATUS_NONFINITE, "step")
+
+    class _Stub:
+        def __init__(self, word):
+            self.word = word
+
+        def read_status(self, device, clear=False):
+            return self.word
+
+    _lib.Geometry.check_finite(_Stub(0), "cpu", "step")
+    with pytest.raises(_lib.VlasovError, match="graph replay"):
+        _lib.Geometry.check_finite(_Stub(1), "cpu", "graph replay of 10 SSP-RK3 steps")

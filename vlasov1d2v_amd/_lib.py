@@ -26,7 +26,7 @@ EXPORTS = (
     "vl_ensemble_get_schedule", "vl_ensemble_moments", "vl_ensemble_rk3_step", "vl_ensemble_autotune",
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
-    "vl_moment_partials", "vl_rk3_step_noalloc",
+    "vl_moment_partials", "vl_rk3_step_noalloc", "vl_read_status",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -107,6 +107,7 @@ def load(path=LIB_PATH):
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
         "vl_moment_partials": [vp, vp, vp, i32, vp],
+        "vl_read_status": [vp, vp, i32, ctypes.POINTER(ctypes.c_uint), vp],
         "vl_rk3_step_noalloc": [vp, vp, vp, vp, vp, vp, f64, vp, i32, vp],
         "vl_ensemble_create": [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
         "vl_ensemble_destroy": [vp],
@@ -153,6 +154,18 @@ def check(rc, what):
     if rc != 0:
         msg = load().vl_last_error_string().decode(errors="replace")
         raise VlasovError(f"{what} failed (code {rc}): {msg}")
+
+
+VL_STATUS_NONFINITE = 1
+
+
+def raise_for_status(status, what):
+    """Raise VlasovError when a workspace status word (vl_read_status) has
+    VL_STATUS_NONFINITE set."""
+    if int(status) & VL_STATUS_NONFINITE:
+        raise VlasovError(f"{what}: the field energy or max|acc E| of a stage-3 field solve is not finite "
+                          "(the state has overflowed -- e.g. a time step above the stability limit -- "
+                          "or holds NaN)")
 
 
 def ptr(t):
@@ -230,6 +243,17 @@ class Geometry:
             self._ws = torch.zeros(max(self.workspace_bytes, 256), dtype=torch.uint8, device=device)
             check(load().vl_workspace_init(self.handle, ptr(self._ws), stream_ptr()), "vl_workspace_init")
         return self._ws
+
+    def read_status(self, device, clear=False):
+        """The status word of this geometry's workspace (synchronises)."""
+        st = ctypes.c_uint()
+        check(load().vl_read_status(self.handle, ptr(self.workspace(device)), 1 if clear else 0, ctypes.byref(st),
+                                    stream_ptr()), "vl_read_status")
+        return int(st.value)
+
+    def check_finite(self, device, what):
+        """Raise VlasovError if a step on this workspace met a non-finite field."""
+        raise_for_status(self.read_status(device), what)
 
     def set_tuning(self, key, value):
         check(load().vl_set_tuning(self.handle, int(key), int(value)), "vl_set_tuning")

@@ -41,6 +41,7 @@ struct EnsRun {
     double *mxpart;
     unsigned *ticket, *seed;
     FieldArgs fa;                  // from_rho = 1 (CFL fields set for adaptive runs)
+    unsigned int *status;          // the run's workspace status word (non-finite field)
     int64_t plane;                 // values per x element of the element-major state
     int n, tpe, inv_blocks, swap_blocks;
     // adaptive time step (main/timestep.py:184,231-242): the run's workspace
@@ -101,6 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_ens_inverse(const EnsRun *__restrict
     const EnsRun &r = runs[b.run];
     FieldArgs fa = r.fa;
     fa.dt_next = (last && r.adaptive) ? r.clock + 2 : nullptr;
+    fa.status = last ? r.status : nullptr;
     dft_inverse_body(r.coef, r.phases, r.wts, r.E, nullptr, r.enpart, r.mxpart, r.ticket,
                      energy ? energy + b.run : nullptr, nullptr, fa, b.local, r.inv_blocks, cdyn, ls);
 }

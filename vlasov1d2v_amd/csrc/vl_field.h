@@ -136,6 +136,11 @@ struct FieldArgs {
     // there, with s0 = max|u|, bv = |acc B max|v||, s2 = |acc |B max|u|||
     double *dt_next;
     double cfl_courant, cfl_s0, cfl_bv, cfl_s2, cfl_dx, cfl_du, cfl_dv;
+    // when set, the inverse sums reduce the field energy and max|acc E| and
+    // set status[0] = 1 (sticky) if either is not finite: the state or the
+    // field has overflowed or holds NaN (the reference has no such check; its
+    // NaN test is commented out, main/timestep.py:118-121)
+    unsigned int *status;
 };
 
 // The reference's adaptive step from max|acc E| (main/timestep.py:231-232,
@@ -296,7 +301,7 @@ __device__ __forceinline__ void dft_inverse_body(
         }
         __syncthreads();
     }
-    const bool want_red = energy || maxabs || a.dt_next;
+    const bool want_red = energy || maxabs || a.dt_next || a.status;
     if (g == 0) {
         DD en;
         double mx = 0.0;
@@ -355,6 +360,9 @@ __device__ __forceinline__ void dft_inverse_body(
         if (energy) energy[0] = 0.5 * tot;
         if (maxabs) maxabs[0] = mx;
         if (a.dt_next) a.dt_next[0] = cfl_dt(a, mx);
+        // fmax drops NaN operands, so the energy (a sum of squares) carries
+        // the check for NaN and the maximum only adds infinities
+        if (a.status && !(isfinite(tot) && isfinite(mx))) a.status[0] = 1u;
         atomicExch(ticket, 0u);
     }
 }

@@ -372,6 +372,8 @@ unsigned int *ws_ticket(const vl_geom *h, void *ws) {
 }
 // rounding seed: the word after the ticket (zeroed with it by vl_workspace_init)
 unsigned int *ws_seed(const vl_geom *h, void *ws) { return ws_ticket(h, ws) + 1; }
+// status word (VL_STATUS_*): the second word after the ticket (zeroed with it)
+unsigned int *ws_status(const vl_geom *h, void *ws) { return ws_ticket(h, ws) + 2; }
 // clock of adaptive runs: (dt, time, next dt)
 double *ws_clock(const vl_geom *h, void *ws) {
     return reinterpret_cast<double *>(static_cast<char *>(ws) + h->off_clock);
@@ -408,12 +410,14 @@ hipError_t launch_reduce(const vl_geom *h, void *ws, double *rho, hipStream_t st
 
 // spectral solve: forward DFT (M/16 workgroups), inverse sums (K/64 workgroups)
 hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho, void *ws, double *E,
-                        double *phi, double *energy, double *maxabs, hipStream_t st, double *dt_next = nullptr) {
+                        double *phi, double *energy, double *maxabs, hipStream_t st, double *dt_next = nullptr,
+                        unsigned *status = nullptr) {
     FieldArgs a = make_field_args(h, from_rho);
     if (dt_next) {
         set_cfl_args(h, a);
         a.dt_next = dt_next;
     }
+    a.status = status;
     const int K = h->nx * h->n;
     const int gf = (h->M + kDftModes - 1) / kDftModes;
     hipLaunchKernelGGL(k_dft_forward, dim3(gf), dim3(kBlock), (size_t)K * sizeof(double), st, rho_or_cd,
@@ -428,10 +432,10 @@ hipError_t launch_field(const vl_geom *h, const double *rho_or_cd, int from_rho,
 
 // partials -> rho (workspace) -> E (+phi, energy)
 hipError_t field_from_partials(const vl_geom *h, void *ws, double *E, double *phi, double *energy,
-                               hipStream_t st, int layout, double *dt_next = nullptr) {
+                               hipStream_t st, int layout, double *dt_next = nullptr, unsigned *status = nullptr) {
     hipError_t e = launch_reduce(h, ws, ws_rho(h, ws), st, layout, ws_seed(h, ws));
     if (e != hipSuccess) return e;
-    return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st, dt_next);
+    return launch_field(h, ws_rho(h, ws), 1, ws, E, phi, energy, nullptr, st, dt_next, status);
 }
 
 hipError_t record_stage_event(const vl_geom *h, int idx, hipStream_t st) {
@@ -945,7 +949,8 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
     VL_HIP(record_stage_event(h, 3, st));
     // stage 3: f = 1/3 f + 2/3 f2 + 2/3 dt L(f2); E, phi, energy of f2's field
     // (and, adaptive, the next dt from that field: main/timestep.py:184)
-    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st, LAY, adaptive ? clock + 2 : nullptr));
+    VL_HIP(field_from_partials(h, ws, E, phi, energy_out, st, LAY, adaptive ? clock + 2 : nullptr,
+                               ws_status(h, ws)));
     s.c0 = 1.0 / 3.0;
     s.c1 = 2.0 / 3.0;
     s.c2 = 2.0 / 3.0;
@@ -1018,6 +1023,20 @@ int vl_rk3_step_noalloc(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, 
     VL_HIP(hipGetLastError());
     return rk3_step(h, fv, f1v, f2v, E, phi, dt, false, energy_out, nullptr, ws, flags, stream,
                     "vl_rk3_step_noalloc");
+}
+
+int vl_read_status(vl_geom_t h, void *ws, int clear, unsigned *status, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!ws || !status) return fail(VL_E_NULL, "vl_read_status: null pointer");
+    if (!aligned(ws, 256)) return fail(VL_E_ALIGN, "vl_read_status: misaligned workspace");
+    hipStream_t st = (hipStream_t)stream;
+    VL_HIP(hipMemcpyAsync(status, ws_status(h, ws), sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    if (clear) {
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, ws_status(h, ws), 1);
+        VL_HIP(hipGetLastError());
+    }
+    VL_HIP(hipStreamSynchronize(st));
+    return VL_OK;
 }
 
 int vl_moment_partials(vl_geom_t h, const void *f, void *ws, int flags, void *stream) {
@@ -1601,6 +1620,7 @@ int vl_ensemble_bind(vl_ensemble_t e, int run, void *f, void *f1, void *f2, doub
     x.ticket = ws_ticket(h, ws);
     x.seed = ws_seed(h, ws);
     x.clock = ws_clock(h, ws);
+    x.status = ws_status(h, ws);
     x.c2[0] = 1.0;
     x.c2[1] = 1.0 / 4.0;
     x.c2[2] = 2.0 / 3.0;

@@ -178,6 +178,14 @@ class Ensemble:
                    "vl_ensemble_clocks")
         return out.cpu().numpy()
 
+    def check_finite(self):
+        """Raise VlasovError naming the runs whose steps met a non-finite
+        field (each run's workspace status; one synchronisation per run)."""
+        bad = [r for r, g in enumerate(self._geoms) if g.read_status(self.device) & _lib.VL_STATUS_NONFINITE]
+        if bad:
+            raise _lib.VlasovError(f"ensemble runs {bad}: the field energy or max|acc E| of a stage-3 field solve "
+                                   "is not finite (the state has overflowed or holds NaN)")
+
     def capture(self, steps):
         """A HIP graph of ``steps`` ensemble steps (step once eagerly first)."""
         return EnsembleGraph(self, steps)
@@ -200,6 +208,7 @@ class Ensemble:
                 self.step(en[k + 1])
                 k += 1
         torch.cuda.synchronize(self.device)
+        self.check_finite()
         return en.cpu().numpy()
 
     def finish(self):
@@ -277,6 +286,7 @@ def time_steps(ens, steps, graph_steps):
         series[2 + k * graph_steps:2 + (k + 1) * graph_steps].copy_(graph.energies)
     t1.record()
     torch.cuda.synchronize(ens.device)
+    ens.check_finite()
     return t0.elapsed_time(t1) / 1e3, reps * graph_steps, series.cpu().numpy()
 
 

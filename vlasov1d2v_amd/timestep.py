@@ -92,7 +92,18 @@ class Stepper:
         self.elapsed = None
         self._res = None          # resident element-major state: dict or None
 
+    # eager steps between two reads of the workspace status in a time loop
+    CHECK_EVERY = 64
+
     # -- helpers -----------------------------------------------------------
+    def check_finite(self, distribution, grids, dg_flux, elliptic):
+        """Raise VlasovError if a step of this stepper met a non-finite field
+        (the workspace's VL_STATUS_NONFINITE bit; one device synchronisation).
+        The time loops call it after every graph replay, every CHECK_EVERY
+        eager steps and at their end; single nonlinear_ssp_rk calls do not."""
+        g, arr, _, _, _ = self._step_state(distribution, grids, dg_flux, elliptic)
+        g.check_finite(arr.device, f"SSP-RK3 step (steps done: {self.steps_counter})")
+
     def _stage_buffers(self, arr):
         import torch
         if (self._stages is None or self._stages[0].shape != arr.shape
@@ -297,6 +308,7 @@ class Stepper:
                 done += 1
                 energy = torch.empty((), dtype=torch.float64, device=dev)
                 self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs, energy_out=energy)
+                self.check_finite(distribution, grids, dg_flux, elliptic)
                 self.time += self.dt
                 self.steps_counter += 1
                 self.time_array = np.append(self.time_array, self.time)
@@ -321,11 +333,14 @@ class Stepper:
                     self.nonlinear_ssp_rk(distribution, basis, elliptic, grids, dg_flux, refs,
                                           energy_out=energies[k + 1:k + 2])
                     n_done = 1
+                    if (k + 1) % self.CHECK_EVERY == 0:
+                        self.check_finite(distribution, grids, dg_flux, elliptic)
                 for _ in range(n_done):
                     self.time += self.dt
                     self.steps_counter += 1
                     self.time_array = np.append(self.time_array, self.time)
                 k += n_done
+            self.check_finite(distribution, grids, dg_flux, elliptic)
             self.resident_end(distribution)
             torch.cuda.synchronize(dev)
             self.field_energy = np.append(self.field_energy, energies.cpu().numpy())
@@ -397,6 +412,9 @@ class Stepper:
                 done += 1
             if steps is None:
                 t_host = float(clocks[-1][-1, 0].item())
+            if not use_graph and (steps is None or done % self.CHECK_EVERY == 0):
+                self.check_finite(distribution, grids, dg_flux, elliptic)
+        self.check_finite(distribution, grids, dg_flux, elliptic)
         self.resident_end(distribution)
         torch.cuda.synchronize(dev)
         if done:
@@ -461,10 +479,18 @@ class StepGraph:
         # the partials in the workspace are still those of the last eager step
         self._mark = Stepper._token(self._arr, self._ws)
 
-    def replay(self):
+    def replay(self, check=True):
+        """Replay the captured steps; with ``check`` then read the workspace
+        status (one synchronisation) and raise VlasovError if a step met a
+        non-finite field."""
         if Stepper._token(self._arr, self._ws) != self._mark:
             _lib.check(_lib.load().vl_moment_partials(self._geom.handle, _lib.ptr(self._arr), _lib.ptr(self._ws),
                                                       self._lay, _lib.stream_ptr()), "vl_moment_partials")
         self.graph.replay()
         self._mark = Stepper._token(self._arr, self._ws)
         self.stepper._rho_token = self._mark
+        if check:
+            self.check()
+
+    def check(self):
+        self._geom.check_finite(self._arr.device, f"graph replay of {self.steps} SSP-RK3 steps")
