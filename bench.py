@@ -81,6 +81,8 @@ def main():
                     help="x elements each stage-kernel workgroup walks (0: the library default)")
     ap.add_argument("--no-mfma", action="store_true",
                     help="run the BF16/FP16 stages on the vector ALUs instead of the matrix cores")
+    ap.add_argument("--rs-kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="row-split stage kernel of the 16-bit forms at n = 9 (0: the library default)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 48-run mixed-shape FP32 ensemble (one persistent launch per RK stage "
                          "for all runs, autotuned schedule), which is timed by default and reported as "
@@ -114,7 +116,7 @@ def main():
     storages = ["f64", "f32", "bf16", "f16"] if args.storage == "all" else [args.storage]
     for storage in storages:
         forms[storage] = measure(w, storage, args.steps, args.warmup, args.layout, dev, mfma=not args.no_mfma,
-                                 xseg=args.xseg)
+                                 xseg=args.xseg, rs_kernel=args.rs_kernel)
         torch.cuda.empty_cache()
     head = forms["f64"] if "f64" in forms else forms[storages[0]]
     N = w.dof
@@ -185,7 +187,7 @@ def profiled_traffic(kernel, storage):
     return (fetch * cal[0] + write * cal[1]) * 1024.0
 
 
-def measure(w, storage, K, W, layout, dev, mfma=True, xseg=0):
+def measure(w, storage, K, W, layout, dev, mfma=True, xseg=0, rs_kernel=0):
     """Time K SSP-RK3 steps (after W warmup steps) of one storage form."""
     import ctypes
     import torch
@@ -197,6 +199,8 @@ def measure(w, storage, K, W, layout, dev, mfma=True, xseg=0):
     g.set_tuning(_lib.VL_TUNE_MFMA, 1 if mfma else 0)
     if xseg > 0:
         g.set_tuning(_lib.VL_TUNE_XSEG, xseg)
+    if rs_kernel > 0:
+        g.set_tuning(_lib.VL_TUNE_RS_KERNEL, rs_kernel)
     energies = torch.empty(W + K + 1, dtype=torch.float64, device=dev)
     if layout == "elem":
         run.resident_begin()

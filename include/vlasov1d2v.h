@@ -117,8 +117,11 @@ int vl_workspace_init(vl_geom_t h, void *workspace, void *stream);
  * smaller than one storage ulp are not lost (round-to-nearest damps the
  * field at small dt); the random bits are a hash of a per-stage seed kept in
  * the workspace and the value's index, so runs are reproducible; 0
- * round-to-nearest-even (as the other 16-bit kernels). */
-enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2, VL_TUNE_ROW_SPLIT = 3, VL_TUNE_ROUNDING = 4 };
+ * round-to-nearest-even (as the other 16-bit kernels).
+ * VL_TUNE_RS_KERNEL: the row-split stage kernel at n = 9: 1 the 9 x 9-frame
+ * kernel (transposes on the matrix core), 2 the natural-frame kernel (LDS
+ * transpose reads, per-direction products, dense epilogue). */
+enum { VL_TUNE_XSEG = 1, VL_TUNE_MFMA = 2, VL_TUNE_ROW_SPLIT = 3, VL_TUNE_ROUNDING = 4, VL_TUNE_RS_KERNEL = 5 };
 int vl_set_tuning(vl_geom_t h, int key, int value);
 int vl_get_tuning(vl_geom_t h, int key, int *value);
 
@@ -181,7 +184,7 @@ int vl_max_abs_acc_e(vl_geom_t h, const double *E, double *out, void *stream);
 int vl_swap_ghosts(vl_geom_t h, void *f, void *stream);
 
 /* One full Shu-Osher SSP-RK3 step of Stepper.nonlinear_ssp_rk
- * (main/timestep.py:141-187) with a fixed dt:
+ * (main/timestep.py:141-187) with a fixed dt > 0 (VL_E_ARG otherwise):
  *   x-ghost swap of f; rho(f) -> E; f1 = f + dt L(f);
  *   rho(f1) -> E; f2 = 3/4 f + 1/4 f1 + 1/4 dt L(f1);
  *   rho(f2) -> E; f = 1/3 f + 2/3 f2 + 2/3 dt L(f2).

@@ -32,6 +32,7 @@ VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
 VL_TUNE_ROW_SPLIT = 3
 VL_TUNE_ROUNDING = 4
+VL_TUNE_RS_KERNEL = 5
 VL_STEP_ELEMENT_MAJOR = 2
 VL_STEP_ROW_SPLIT = 4
 VL_LAYOUT_REFERENCE, VL_LAYOUT_ELEMENT_MAJOR, VL_LAYOUT_ROW_SPLIT = 0, 1, 2

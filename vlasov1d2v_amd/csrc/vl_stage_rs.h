@@ -158,6 +158,28 @@ __device__ __forceinline__ void wave_sync_lds() {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Vector-memory loads one lane issues for the next tile's prefetch: per own
+// row its 16 bytes, the two v-face words and (n = 9) the tail word, then the
+// neighbour row and (n = 9) its tail.  Every load is issued, masked lanes with
+// an out-of-range offset, so the count is fixed.
+template <int N>
+constexpr int rs_prefetch_loads() {
+    return (N == 9 ? 4 : 3) * ((N * N + 63) / 64) + (N == 9 ? 2 : 1);
+}
+
+// s_waitcnt through the builtin, so that the compiler's own wait insertion
+// knows which loads have landed: vector-memory count K, no lgkm / export wait
+// (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int K>
+__device__ __forceinline__ void rs_wait_vm() {
+    static_assert(K >= 0 && K < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((K & 15) | ((K >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// Every outstanding load landed.  Issued once before a tile loop: loads of
+// per-lane constants would otherwise be waited for at their first use inside
+// the loop, where the count also covers the next tile's prefetch.
+__device__ __forceinline__ void rs_wait_all() { __builtin_amdgcn_s_waitcnt(0); }
+
 template <int N>
 struct RsLoads {
     vl_u32x4 row[RsShape<N>::NRI];   // own rows: nodes 0..7 of line (ix, iu)
@@ -376,6 +398,15 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
 
     // Per tile: fill the brick from `cur`, prefetch the next tile into `nxt`,
     // compute, store.
+    // moment weight of each own row's u-node (rows ln + 64 k)
+    float wuk[NRI];
+#pragma unroll
+    for (int k = 0; k < NRI; ++k) {
+        const int r = ln + 64 * k;
+        wuk[k] = r < ROWS ? ((const float *)tabp->wu)[r - (r / N) * N] : 0.f;
+    }
+    rs_wait_all();
+
     RsLoads<N> cur, nxt;
     load(pc.ex_begin, cur);
     for (int ex = pc.ex_begin; ex <= pc.ex_end; ++ex) {
@@ -460,7 +491,12 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             // ---- epilogue: whole rows ----------------------------------------
             const int64_t bo = ebase(ex, eu, ev);
             if (MODE == kModeStage) {
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the staged f^n is in LDS
+                // the staged f^n is in LDS: its loads were issued before the next tile's
+                // prefetch, which may stay in flight
+                if (has_next)
+                    rs_wait_vm<rs_prefetch_loads<N>()>();
+                else
+                    rs_wait_vm<0>();
             }
 #pragma unroll 1
             for (int k = 0; k < NRI; ++k) {
@@ -522,7 +558,11 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                         m += tab.wv[8] * PK::lo(t8);
                     }
                     // moment partial of the stored values (main/grid.py:319-321)
-                    red[el * ROWS + r] = tab.wu[iu] * m;
+                    float wr = wuk[0];
+#pragma unroll
+                    for (int q = 1; q < NRI; ++q)
+                        if (k == q) wr = wuk[q];
+                    red[el * ROWS + r] = wr * m;
                 }
             }
         }

@@ -18,6 +18,7 @@
 #include "vl_stage.h"
 #include "vl_stage_mx.h"
 #include "vl_stage_rs.h"
+#include "vl_stage_rs2.h"
 #include "vl_ensemble.h"
 #include "vl_init.h"
 #include "vl_central.h"
@@ -38,6 +39,7 @@ struct vl_geom {
     int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: 4 v-elements per tile
     int xseg4;             // row-split stage kernel: x elements per pencil
     int rounding;          // 16-bit forms, row-split kernel: 0 nearest-even, 1 stochastic
+    int rs_gen;            // row-split stage kernel at n = 9: 1 k_stage_rs, 2 k_stage_rs2
     int64_t sxe_elem;      // values per x element in the element-major layout
     int64_t sx, sxe, su, sue;
     size_t state_elems;
@@ -251,6 +253,19 @@ hipError_t launch_stage_mx(const vl_geom *h, const void *fin, const void *fgh, c
 template <int N, int MODE, int S>
 hipError_t launch_stage_rs_ns(const vl_geom *h, const void *fin, const void *fgh, const void *fn, void *fout,
                               const double *E, const StageArgs &s, hipStream_t st) {
+    if constexpr (N == 9) {
+        if (h->rs_gen == 2) {
+            if (s.dt_dev)
+                hipLaunchKernelGGL((k_stage_rs2<N, MODE, S, true>), dim3(h->pencils4), dim3(kBlock), 0, st,
+                                   (const uint16_t *)fin, (const uint16_t *)fgh, (const uint16_t *)fn,
+                                   (uint16_t *)fout, E, h->d_unodes, h->d_vnodes, s, h->d_tab);
+            else
+                hipLaunchKernelGGL((k_stage_rs2<N, MODE, S>), dim3(h->pencils4), dim3(kBlock), 0, st,
+                                   (const uint16_t *)fin, (const uint16_t *)fgh, (const uint16_t *)fn,
+                                   (uint16_t *)fout, E, h->d_unodes, h->d_vnodes, s, h->d_tab);
+            return hipGetLastError();
+        }
+    }
     if (s.dt_dev)
         hipLaunchKernelGGL((k_stage_rs<N, MODE, S, true>), dim3(h->pencils4), dim3(kBlock), 0, st,
                            (const uint16_t *)fin, (const uint16_t *)fgh, (const uint16_t *)fn, (uint16_t *)fout, E,
@@ -706,6 +721,7 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     // row-split kernels: tiles of 4 v-elements (same x-segments)
     h->row_split = 1;
     h->rounding = 1;
+    h->rs_gen = 1;
     h->nchunk4 = (d->nv + 3) / 4;
     h->tiles_per_ex4 = d->nu * h->nchunk4;
     h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
@@ -899,6 +915,7 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
     if (!aligned(fv, eb) || !aligned(f1v, eb) || !aligned(f2v, eb) || !aligned(E, 8) || !aligned(ws, 256))
         return fail(VL_E_ALIGN, "%s: misaligned pointer", what);
     if (!std::isfinite(dt)) return fail(VL_E_NOTFINITE, "%s: dt not finite", what);
+    if (!adaptive && !(dt > 0.0)) return fail(VL_E_ARG, "%s: dt must be > 0", what);
     if (adaptive && !h->cfl_set) return fail(VL_E_ARG, "%s: adaptive steps need vl_set_cfl first", what);
     if (adaptive && !(flags & (VL_STEP_ELEMENT_MAJOR | VL_STEP_ROW_SPLIT)))
         return fail(VL_E_ARG, "%s: adaptive steps run in a resident layout (VL_STEP_ELEMENT_MAJOR / ROW_SPLIT)", what);
@@ -1135,6 +1152,11 @@ int vl_set_tuning(vl_geom_t h, int key, int value) {
         h->mfma = value;
         return VL_OK;
     }
+    if (key == VL_TUNE_RS_KERNEL) {
+        if (value != 1 && value != 2) return fail(VL_E_ARG, "row-split kernel must be 1 or 2");
+        h->rs_gen = value;
+        return VL_OK;
+    }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
 }
 
@@ -1155,6 +1177,10 @@ int vl_get_tuning(vl_geom_t h, int key, int *value) {
     }
     if (key == VL_TUNE_ROUNDING) {
         *value = h->rounding;
+        return VL_OK;
+    }
+    if (key == VL_TUNE_RS_KERNEL) {
+        *value = h->rs_gen;
         return VL_OK;
     }
     return fail(VL_E_ARG, "unknown tuning key %d", key);
