@@ -3,8 +3,8 @@
 
 ``poisson2`` is the spectral (high-order nodal DFT) solve of the hot path.
 ``poisson`` is the reference's stabilized central-flux DG solve: its operator
-is assembled and inverted on the host exactly as the reference does
-(``build_central_flux_operator`` / ``invert``), then applied on the device.
+is assembled (as a product of block operators) and inverted on the host, then
+applied on the device.
 """
 import numpy as np
 
@@ -62,39 +62,60 @@ class Elliptic:
         return out
 
     # -- stabilized central-flux DG Poisson (main/elliptic.py:24-115) ---------
+    @staticmethod
+    def _face_jumps(res, n):
+        """Face-jump matrix F, shape (2 res, res n): row (e, 0) is the jump
+        phi_e(left node) - phi_{e-1}(right node) at element e's left face, row
+        (e, 1) the jump phi_e(right node) - phi_{e+1}(left node) at its right
+        face (periodic element neighbours)."""
+        F = np.zeros((res, 2, res, n))
+        e = np.arange(res)
+        F[e, 0, e, 0] += 1.0
+        F[e, 0, (e - 1) % res, n - 1] -= 1.0
+        F[e, 1, e, n - 1] += 1.0
+        F[e, 1, (e + 1) % res, 0] -= 1.0
+        return F.reshape(2 * res, res * n)
+
     def build_central_flux_operator(self, grid, basis):
-        """Assemble the central-flux DG Laplacian column by column from nodal
-        indicators, with the gauge row/column (zero quadrature mean of the
-        potential), as main/elliptic.py:24-81 does (host, FP64)."""
-        res, order = grid.res, grid.order
-        indicator = np.zeros((res, order))
-        face_diff0 = np.zeros((res, 2))
-        face_diff1 = np.zeros((res, 2))
-        num_flux = np.zeros_like(face_diff0)
-        grad_num_flux = np.zeros_like(face_diff1)
-        op = np.zeros((res, order, res, order))
-        self.gradient_operator = np.zeros_like(op)
-        for i in range(res):
-            for j in range(order):
-                indicator[i, j] = 1.0
-                face_diff0[:, 0] = indicator[:, 0] - np.roll(indicator[:, -1], 1)
-                face_diff0[:, 1] = indicator[:, -1] - np.roll(indicator[:, 0], -1)
-                num_flux[:, 0] = 0.5 * face_diff0[:, 0]
-                num_flux[:, 1] = -0.5 * face_diff0[:, 1]
-                grad = (np.tensordot(basis.der, indicator, axes=([1], [1])) +
-                        np.tensordot(basis.np_xi, num_flux, axes=([1], [1]))).T
-                face_diff1[:, 0] = grad[:, 0] - np.roll(grad[:, -1], 1)
-                face_diff1[:, 1] = grad[:, -1] - np.roll(grad[:, 0], -1)
-                grad_num_flux[:, 0] = 0.5 * face_diff1[:, 0]
-                grad_num_flux[:, 1] = -0.5 * face_diff1[:, 1]
-                op[i, j] = (np.tensordot(basis.stf, grad, axes=([1], [1])) +
-                            np.tensordot(basis.face_mass, grad_num_flux + face_diff0, axes=([1], [1]))).T
-                self.gradient_operator[i, j] = grad
-                indicator[i, j] = 0
-        qw = np.asarray(grid.quad_weights, dtype=np.float64)
-        op0 = np.hstack([op.reshape(res * order, res * order), qw.reshape(res * order, 1)])
-        self.central_flux_operator = np.vstack([op0, np.append(qw.flatten(), 0)])
-        self.central_flux_operator[np.abs(self.central_flux_operator) < 1.0e-15] = 0
+        """Assemble the central-flux DG Laplacian of main/elliptic.py:24-81 as a
+        product of block operators (host, FP64).
+
+        With F the face jumps and s = (+1/2, -1/2) the central-flux sign per
+        face, the DG gradient is G = I (x) D + (I (x) Xi) diag(s) F, and the
+        operator is L = (I (x) S^T) G + (I (x) M_f) (diag(s) F G + F): stiffness
+        against the gradient plus the face terms of the gradient's central flux
+        and of the potential's jump penalty.  The reference stores both
+        operators input-major (row = perturbed node, column = response node),
+        so ``gradient_operator`` holds G^T as (res, n, res, n), and
+        ``central_flux_operator`` is [[L^T, w], [w^T, 0]] with the quadrature
+        weights w fixing the gauge (zero mean potential); entries below 1e-15
+        in magnitude are set to zero, as the reference does."""
+        res, n = grid.res, grid.order
+        K = res * n
+        D = np.asarray(basis.der, dtype=np.float64)           # (n, n) nodal derivative
+        Xi = np.asarray(basis.np_xi, dtype=np.float64)        # (n, 2) face lift
+        St = np.asarray(basis.stf, dtype=np.float64)          # (n, n) stiffness (transposed)
+        Mf = np.asarray(basis.face_mass, dtype=np.float64)    # (n, 2) face mass
+        F = self._face_jumps(res, n)
+        sF = np.tile([0.5, -0.5], res)[:, None] * F           # central numerical flux of a field
+
+        def per_element(A, B):
+            """(I (x) A) B for A of shape (n, m) and B of shape (res m, K)."""
+            m = A.shape[1]
+            return np.einsum("am,emk->eak", A, B.reshape(res, m, K)).reshape(K, K)
+
+        G = per_element(Xi, sF)
+        e = np.arange(res)
+        G.reshape(res, n, res, n)[e, :, e, :] += D            # the element-local derivative blocks
+        L = per_element(St, G) + per_element(Mf, sF @ G + F)
+        self.gradient_operator = np.ascontiguousarray(G.T).reshape(res, n, res, n)
+        w = np.asarray(grid.quad_weights, dtype=np.float64).reshape(K)
+        op = np.zeros((K + 1, K + 1))
+        op[:K, :K] = L.T
+        op[:K, K] = w
+        op[K, :K] = w
+        op[np.abs(op) < 1.0e-15] = 0.0
+        self.central_flux_operator = op
         self._cf_geom = None
 
     def invert(self):
