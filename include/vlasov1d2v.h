@@ -294,6 +294,44 @@ int vl_set_background(vl_geom_t h, const double *f0_host);
 int vl_export_f32(vl_geom_t h, const void *f, float *out, void *stream);
 int vl_import_f32(vl_geom_t h, const float *in, void *f, void *stream);
 
+/* Element-wise passes over whole states (reference layout, ghosts included),
+ * so that tensors stay buffers on every product path.
+ *
+ * vl_axpby: y = a y + b x on every node of an FP64 state of the handle's shape,
+ * or (interior != 0) only on the nodes of interior elements, leaving the
+ * ghost planes of y as they are.  a == 0 does not read y, b == 0 does not
+ * read x (x may then be NULL).  The combinations of the linear SSP-RK update,
+ * main/timestep.py:189-229 (`arr[ng] = c0 arr[ng]`, `copy[ng] += dt rhs[ng]`,
+ * `arr[ng] += c_i copy[ng]`), one fused multiply-add per node; with a = 0,
+ * b = 1 and interior = 0 the copy of main/grid.py:198-199.
+ *
+ * vl_init_separable: f[X, (u, v)] = xprofile[X] vel[(u, v)] for the FP64 form
+ * (the ring of main/grid.py:244-279 when xprofile is one and vel the ring);
+ * the reduced forms store g = (xprofile[X] vel[(u, v)]) scale, the deviation
+ * f - f0 of a state whose deviation is separable (xprofile = modulation - 1).
+ * xprofile has (NX+2) n values, vel (NU+2) n (NV+2) n; both on the device.
+ * Rounded to nearest even (16-bit forms through FP32).
+ *
+ * vl_deviation_to_storage (reduced forms): g = dev scale rounded to the
+ * storage type, dev = f - f0 in FP64.
+ *
+ * vl_total_f64: out = f (FP64 form) or f0 + g / scale (reduced forms, which
+ * need vl_set_background first), FP64, the whole ghosted state.
+ *
+ * vl_density_deviation: out = mean(rho) - rho (sign = 1), the charge density
+ * the time steps hand to the Poisson solve (main/timestep.py:155,172,203,221,
+ * `cp.mean(density) - density`), or rho - mean(rho) (sign = -1, the initial
+ * solve of main/main.py:131); rho and out are (NX, n) FP64, mean the plain
+ * mean over all NX n values (summed in a fixed order).
+ *
+ * No counterparts in the reference beyond the cited statements: it forms
+ * these with CuPy array arithmetic. */
+int vl_axpby(vl_geom_t h, double *y, const double *x, double a, double b, int interior, void *stream);
+int vl_init_separable(vl_geom_t h, const double *vel, const double *xprofile, void *f, void *stream);
+int vl_deviation_to_storage(vl_geom_t h, const double *dev, void *g, void *stream);
+int vl_total_f64(vl_geom_t h, const void *f, double *out, void *stream);
+int vl_density_deviation(vl_geom_t h, const double *rho, double *out, int sign, void *stream);
+
 /* ---- Ensembles (no counterpart in the reference, which steps one run per
  * process through main/main.py:165 -> Stepper.main_loop).  Many independent
  * runs -- each its own geometry (n, grid, physics), state and workspace --

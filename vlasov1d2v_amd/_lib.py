@@ -27,6 +27,7 @@ EXPORTS = (
     "vl_set_cfl", "vl_clock_init", "vl_rk3_step_adaptive", "vl_init_eigen", "vl_set_central_flux",
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
     "vl_moment_partials", "vl_rk3_step_noalloc", "vl_read_status",
+    "vl_axpby", "vl_init_separable", "vl_deviation_to_storage", "vl_total_f64", "vl_density_deviation",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -104,6 +105,11 @@ def load(path=LIB_PATH):
         "vl_poisson_central": [vp, vp, vp, vp, i32, vp],
         "vl_export_f32": [vp, vp, vp, vp],
         "vl_import_f32": [vp, vp, vp, vp],
+        "vl_axpby": [vp, vp, vp, f64, f64, i32, vp],
+        "vl_init_separable": [vp, vp, vp, vp, vp],
+        "vl_deviation_to_storage": [vp, vp, vp, vp],
+        "vl_total_f64": [vp, vp, vp, vp],
+        "vl_density_deviation": [vp, vp, vp, i32, vp],
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
@@ -138,7 +144,8 @@ def load(path=LIB_PATH):
 WRITERS = frozenset({
     "vl_moment0", "vl_moment_partials", "vl_rk3_step", "vl_rk3_step_adaptive", "vl_rk3_step_noalloc",
     "vl_convert_layout",
-    "vl_import_f32", "vl_init_eigen", "vl_ensemble_rk3_step", "vl_ensemble_moments", "vl_ensemble_autotune",
+    "vl_import_f32", "vl_init_eigen", "vl_axpby", "vl_init_separable", "vl_deviation_to_storage",
+    "vl_ensemble_rk3_step", "vl_ensemble_moments", "vl_ensemble_autotune",
 })
 _generation = 0
 
@@ -176,6 +183,26 @@ def ptr(t):
     if isinstance(t, np.ndarray):
         return t.ctypes.data
     return t.data_ptr()
+
+
+def axpby(geom, y, x, a, b, interior=True):
+    """y = a y + b x on the (interior) nodes of an FP64 state (vl_axpby)."""
+    check(load().vl_axpby(geom.handle, ptr(y), ptr(x), float(a), float(b), 1 if interior else 0, stream_ptr()),
+          "vl_axpby")
+
+
+def density_deviation(geom, rho, sign=1):
+    """mean(rho) - rho (sign 1) or rho - mean(rho) (sign -1), a new (NX, n)
+    FP64 GPU tensor (vl_density_deviation)."""
+    import torch
+    if not isinstance(rho, torch.Tensor) or rho.device.type != "cuda" or rho.dtype != torch.float64 \
+            or not rho.is_contiguous() or tuple(rho.shape) != (geom.nx, geom.n):
+        raise ValueError(f"density_deviation: rho must be a contiguous float64 GPU tensor of shape "
+                         f"{(geom.nx, geom.n)}")
+    out = torch.empty_like(rho)
+    check(load().vl_density_deviation(geom.handle, ptr(rho), ptr(out), int(sign), stream_ptr()),
+          "vl_density_deviation")
+    return out
 
 
 def stream_ptr(stream=None):

@@ -23,6 +23,7 @@
 #include "vl_init.h"
 #include "vl_central.h"
 #include "vl_export.h"
+#include "vl_elementwise.h"
 
 #include <vector>
 
@@ -1376,6 +1377,126 @@ int vl_import_f32(vl_geom_t h, const float *in, void *f, void *stream) {
     default:
         hipLaunchKernelGGL(k_import_f32<3>, grid, dim3(kBlock), 0, st, in, h->d_f0, (uint16_t *)f, plane,
                            h->scale);
+        break;
+    }
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+// ---- element-wise passes over whole states ---------------------------------
+
+static PlaneShape plane_shape(const vl_geom *h) {
+    PlaneShape p;
+    p.row = (h->nv + 2) * h->n;
+    p.plane = (int64_t)(h->nu + 2) * h->n * p.row;
+    p.n = h->n;
+    p.nu = h->nu;
+    p.nv = h->nv;
+    return p;
+}
+
+int vl_axpby(vl_geom_t h, double *y, const double *x, double a, double b, int interior, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!y || (b != 0.0 && !x)) return fail(VL_E_NULL, "vl_axpby: null pointer");
+    if (!aligned(y, 8) || (x && !aligned(x, 8))) return fail(VL_E_ALIGN, "vl_axpby: misaligned");
+    if (!std::isfinite(a) || !std::isfinite(b)) return fail(VL_E_NOTFINITE, "vl_axpby: non-finite coefficient");
+    const PlaneShape p = plane_shape(h);
+    const int planes = (interior ? h->nx : h->nx + 2) * h->n;
+    const dim3 grid((unsigned)((p.plane + kBlock - 1) / kBlock), (unsigned)planes);
+    hipLaunchKernelGGL(k_axpby, grid, dim3(kBlock), 0, (hipStream_t)stream, y, x, a, b, p, interior ? h->n : 0,
+                       interior ? 1 : 0);
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_init_separable(vl_geom_t h, const double *vel, const double *xprofile, void *f, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!vel || !xprofile || !f) return fail(VL_E_NULL, "vl_init_separable: null pointer");
+    if (!aligned(vel, 8) || !aligned(xprofile, 8) || !aligned(f, storage_bytes(h->storage)))
+        return fail(VL_E_ALIGN, "vl_init_separable: misaligned");
+    const PlaneShape p = plane_shape(h);
+    const dim3 grid((unsigned)((p.plane + kBlock - 1) / kBlock), (unsigned)((h->nx + 2) * h->n));
+    hipStream_t st = (hipStream_t)stream;
+    switch (h->storage) {
+    case VL_F64:
+        hipLaunchKernelGGL(k_init_separable<0>, grid, dim3(kBlock), 0, st, (double *)f, vel, xprofile, 1.0,
+                           p.plane);
+        break;
+    case VL_F32:
+        hipLaunchKernelGGL(k_init_separable<1>, grid, dim3(kBlock), 0, st, (float *)f, vel, xprofile, h->scale,
+                           p.plane);
+        break;
+    case VL_BF16:
+        hipLaunchKernelGGL(k_init_separable<2>, grid, dim3(kBlock), 0, st, (uint16_t *)f, vel, xprofile, h->scale,
+                           p.plane);
+        break;
+    default:
+        hipLaunchKernelGGL(k_init_separable<3>, grid, dim3(kBlock), 0, st, (uint16_t *)f, vel, xprofile, h->scale,
+                           p.plane);
+        break;
+    }
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_deviation_to_storage(vl_geom_t h, const double *dev, void *g, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!dev || !g) return fail(VL_E_NULL, "vl_deviation_to_storage: null pointer");
+    if (h->storage == VL_F64) return fail(VL_E_ARG, "vl_deviation_to_storage: FP64 storage keeps f itself");
+    if (!aligned(dev, 8) || !aligned(g, storage_bytes(h->storage)))
+        return fail(VL_E_ALIGN, "vl_deviation_to_storage: misaligned");
+    const int64_t count = plane_shape(h).plane * (h->nx + 2) * h->n;
+    const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));
+    hipStream_t st = (hipStream_t)stream;
+    switch (h->storage) {
+    case VL_F32:
+        hipLaunchKernelGGL(k_deviation_to_storage<1>, grid, dim3(kBlock), 0, st, dev, (float *)g, h->scale, count);
+        break;
+    case VL_BF16:
+        hipLaunchKernelGGL(k_deviation_to_storage<2>, grid, dim3(kBlock), 0, st, dev, (uint16_t *)g, h->scale,
+                           count);
+        break;
+    default:
+        hipLaunchKernelGGL(k_deviation_to_storage<3>, grid, dim3(kBlock), 0, st, dev, (uint16_t *)g, h->scale,
+                           count);
+        break;
+    }
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_density_deviation(vl_geom_t h, const double *rho, double *out, int sign, void *stream) {
+    if (int r = check_handle(h)) return r;
+    if (!rho || !out) return fail(VL_E_NULL, "vl_density_deviation: null pointer");
+    if (!aligned(rho, 8) || !aligned(out, 8)) return fail(VL_E_ALIGN, "vl_density_deviation: misaligned");
+    if (sign != 1 && sign != -1) return fail(VL_E_ARG, "vl_density_deviation: sign must be 1 or -1");
+    hipLaunchKernelGGL(k_density_deviation, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, rho, out, h->nx * h->n,
+                       sign);
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
+int vl_total_f64(vl_geom_t h, const void *f, double *out, void *stream) {
+    if (int r = record_check(h, f, out, "vl_total_f64")) return r;
+    if (!aligned(f, storage_bytes(h->storage)) || !aligned(out, 8)) return fail(VL_E_ALIGN, "vl_total_f64: misaligned");
+    const PlaneShape p = plane_shape(h);
+    const dim3 grid((unsigned)((p.plane + kBlock - 1) / kBlock), (unsigned)((h->nx + 2) * h->n));
+    hipStream_t st = (hipStream_t)stream;
+    switch (h->storage) {
+    case VL_F64:
+        hipLaunchKernelGGL(k_total_f64<0>, grid, dim3(kBlock), 0, st, (const double *)f, h->d_f0, out, p.plane, 1.0);
+        break;
+    case VL_F32:
+        hipLaunchKernelGGL(k_total_f64<1>, grid, dim3(kBlock), 0, st, (const float *)f, h->d_f0, out, p.plane,
+                           h->inv_scale);
+        break;
+    case VL_BF16:
+        hipLaunchKernelGGL(k_total_f64<2>, grid, dim3(kBlock), 0, st, (const uint16_t *)f, h->d_f0, out, p.plane,
+                           h->inv_scale);
+        break;
+    default:
+        hipLaunchKernelGGL(k_total_f64<3>, grid, dim3(kBlock), 0, st, (const uint16_t *)f, h->d_f0, out, p.plane,
+                           h->inv_scale);
         break;
     }
     VL_HIP(hipGetLastError());

@@ -214,12 +214,22 @@ class Distribution:
             if self.quad_weights is None:
                 self.initialize_quad_weights(grids)
             return
-        ring = torch.from_numpy(self.ring(grids)).to(self.device)
-        mod = np.ones_like(grids.x.arr) if modulation is None else np.asarray(modulation)
-        mod = torch.from_numpy(np.ascontiguousarray(mod)).to(self.device)
-        self.arr = (ring[None, None, :, :, :, :] * mod[:, :, None, None, None, None]).contiguous()
+        self.arr = torch.empty(grids.shape, dtype=torch.float64, device=self.device)
+        self._separable(grids.geometry(), self.ring(grids), modulation, self.arr)
         if self.quad_weights is None:
             self.initialize_quad_weights(grids)
+
+    def _separable(self, geom, vel, xprofile, out):
+        """out = xprofile(x) vel(u, v) in geom's storage form (vl_init_separable);
+        vel (NU+2, n, NV+2, n) and xprofile (NX+2, n) host arrays, uploaded as buffers."""
+        import torch
+        vel_d = torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float64)).to(self.device)
+        xp_d = torch.from_numpy(np.ascontiguousarray(xprofile, dtype=np.float64)).to(self.device)
+        if vel_d.numel() != (geom.nu + 2) * geom.n * (geom.nv + 2) * geom.n or \
+                xp_d.numel() != (geom.nx + 2) * geom.n:
+            raise ValueError("separable state: table sizes do not match the geometry")
+        _lib.check(_lib.load().vl_init_separable(geom.handle, _lib.ptr(vel_d), _lib.ptr(xp_d), _lib.ptr(out),
+                                                 _lib.stream_ptr()), "vl_init_separable")
 
     def _eigen_state(self, grids, pert_only):
         """ring + eigenfunction (or the eigenfunction alone) as an FP64 GPU tensor."""
@@ -262,15 +272,16 @@ class Distribution:
         self.scale = STORAGE_SCALE[storage]
         ring_h = self.ring(grids)
         self.f0 = ring_h
-        if ic == "eigen":
-            g = self._eigen_state(grids, pert_only=True)   # f - f0: the eigenfunction alone
-        else:
-            ring = torch.from_numpy(ring_h).to(self.device)
-            pert = torch.from_numpy(np.ascontiguousarray(self._pert)).to(self.device)
-            g = ring[None, None, :, :, :, :] * pert[:, :, None, None, None, None]
-        self.arr = (g * self.scale).to(storage_dtype(storage)).contiguous()
-        del g
         geo = grids.geometry(STORAGE_CODES[storage])
+        out = torch.empty(grids.shape, dtype=storage_dtype(storage), device=self.device)
+        if ic == "eigen":
+            d = self._eigen_state(grids, pert_only=True)   # f - f0: the eigenfunction alone
+            _lib.check(_lib.load().vl_deviation_to_storage(geo.handle, _lib.ptr(d), _lib.ptr(out),
+                                                           _lib.stream_ptr()), "vl_deviation_to_storage")
+            del d
+        else:
+            self._separable(geo, ring_h, self._pert, out)  # f - f0 = ring (modulation - 1)
+        self.arr = out
         _lib.check(_lib.load().vl_set_background(geo.handle, _lib.ptr(np.ascontiguousarray(ring_h))),
                    "vl_set_background")
 
@@ -279,8 +290,11 @@ class Distribution:
         import torch
         if self.storage == "f64":
             return self.arr
-        f0 = torch.from_numpy(self.f0).to(self.arr.device)
-        return f0[None, None] + self.arr.to(torch.float64) / self.scale
+        geo = self._grids.geometry(STORAGE_CODES[self.storage])
+        out = torch.empty(tuple(self.arr.shape), dtype=torch.float64, device=self.arr.device)
+        _lib.check(_lib.load().vl_total_f64(geo.handle, _lib.ptr(self.arr.contiguous()), _lib.ptr(out),
+                                            _lib.stream_ptr()), "vl_total_f64")
+        return out
 
     def _geom(self):
         if self._grids is None:

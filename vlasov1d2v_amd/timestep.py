@@ -230,24 +230,25 @@ class Stepper:
         if self.dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
         c = self.coefficients
-        ng = grids.no_ghost_slice
         geo = distribution._geom()
         lib = _lib.load()
 
         def field_of(copy):
             _lib.check(lib.vl_swap_ghosts(geo.handle, _lib.ptr(copy), _lib.stream_ptr()), "vl_swap_ghosts")
             density = distribution.moment_zero_of_arr(copy)
-            elliptic.poisson(density.mean() - density, grids.x, basis.b1)
+            elliptic.poisson(_lib.density_deviation(geo, density), grids.x, basis.b1)
 
+        import torch
         distribution.swap_ghost_cells_of_array()
-        copy = arr.clone()                                  # copy_for_explicit_advance (main/grid.py:198-199)
-        arr[ng] = c[0] * arr[ng]
+        copy = torch.empty_like(arr)
+        _lib.axpby(geo, copy, arr, 0.0, 1.0, interior=False)    # copy_for_explicit_advance (main/grid.py:198-199)
+        _lib.axpby(geo, arr, None, c[0], 0.0)                    # arr[ng] = c0 arr[ng]
         for i in range(1, self.time_order):
             field_of(copy)
-            copy[ng] += self.dt * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
-            arr[ng] += c[i] * copy[ng]
+            _lib.axpby(geo, copy, dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids), 1.0, self.dt)
+            _lib.axpby(geo, arr, copy, 1.0, c[i])
         field_of(copy)
-        arr[ng] += (c[-1] * self.dt) * dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids)[ng]
+        _lib.axpby(geo, arr, dg_flux.semi_discrete_rhs(copy, elliptic, basis, grids), 1.0, c[-1] * self.dt)
         self._rho_token = None
         if self.fixed_dt is None:
             self._adapt(grids, elliptic, dg_flux, refs)
@@ -257,25 +258,27 @@ class Stepper:
         reference's order (main/timestep.py:141-187)."""
         import torch
         arr = distribution.arr
+        geo = distribution._geom()
+        lib = _lib.load()
         distribution.swap_ghost_cells_of_array()
-        stages = torch.zeros((self.time_order,) + tuple(arr.shape), dtype=arr.dtype, device=arr.device)
-        stages[:, :, :, 0] = arr[:, :, 0]
-        stages[:, :, :, -1] = arr[:, :, -1]
-        stages[:, :, :, :, :, 0] = arr[:, :, :, :, 0]
-        stages[:, :, :, :, :, -1] = arr[:, :, :, :, -1]
-        ng = grids.no_ghost_slice
+        # every stage starts as a copy of f^n: its u/v ghost planes are the
+        # frozen ones of f^n; its x ghosts are swapped before use
+        stages = torch.empty((self.time_order,) + tuple(arr.shape), dtype=arr.dtype, device=arr.device)
+        for i in range(self.time_order):
+            _lib.axpby(geo, stages[i], arr, 0.0, 1.0, interior=False)
         density = distribution.moment_zero()
-        elliptic.poisson2(torch.mean(density) - density, grids.x, basis.b1)
-        stages[(0,) + ng] = arr[ng] + self.dt * dg_flux.semi_discrete_rhs(arr, elliptic, basis, grids)[ng]
+        elliptic.poisson2(_lib.density_deviation(geo, density), grids.x, basis.b1)
+        _lib.axpby(geo, stages[0], dg_flux.semi_discrete_rhs(arr, elliptic, basis, grids), 1.0, self.dt)
         for i in range(1, self.time_order):
-            stages[i - 1, 0] = stages[i - 1, -2]
-            stages[i - 1, -1] = stages[i - 1, 1]
+            _lib.check(lib.vl_swap_ghosts(geo.handle, _lib.ptr(stages[i - 1]), _lib.stream_ptr()), "vl_swap_ghosts")
             density = distribution.moment_zero_of_arr(stages[i - 1])
-            elliptic.poisson2(torch.mean(density) - density, grids.x, basis.b1)
-            df_dt = dg_flux.semi_discrete_rhs(stages[i - 1].contiguous(), elliptic, basis, grids)[ng]
+            elliptic.poisson2(_lib.density_deviation(geo, density), grids.x, basis.b1)
+            df_dt = dg_flux.semi_discrete_rhs(stages[i - 1], elliptic, basis, grids)
             c = self.coefficients[i - 1]
-            stages[(i,) + ng] = c[0] * arr[ng] + c[1] * stages[(i - 1,) + ng] + c[2] * self.dt * df_dt
-        arr[ng] = stages[-1][ng]
+            _lib.axpby(geo, stages[i], arr, 0.0, c[0])
+            _lib.axpby(geo, stages[i], stages[i - 1], 1.0, c[1])
+            _lib.axpby(geo, stages[i], df_dt, 1.0, c[2] * self.dt)
+        _lib.axpby(geo, arr, stages[-1], 0.0, 1.0)
         self._rho_token = None
 
     # -- time loop -------------------------------------------------------------

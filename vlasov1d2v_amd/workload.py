@@ -11,6 +11,8 @@ import dataclasses
 
 import numpy as np
 
+from . import _lib
+
 from .basis import Basis3D
 from .elliptic import Elliptic
 from .fluxes import DGFlux
@@ -78,7 +80,8 @@ class Run:
         self.flux = DGFlux(np.array([nx + 2, nu + 2, nv + 2]), np.array([n, n, n]), acc)
         self.flux._geom(self.grids, self.elliptic)
         n0 = self.dist.moment_zero()
-        self.elliptic.poisson2(n0 - n0.mean(), self.grids.x, self.basis.b1)   # main/main.py:131
+        self.elliptic.poisson2(_lib.density_deviation(self.grids.geometry(), n0, sign=-1),
+                               self.grids.x, self.basis.b1)                   # main/main.py:131
         self.storage = storage
         if storage != "f64":
             self.dist.to_storage(self.grids, storage)
