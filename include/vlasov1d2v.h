@@ -343,6 +343,22 @@ int vl_density_deviation(vl_geom_t h, const double *rho, double *out, int sign, 
  * compiler's multiply-add fusion (rounding level); the potential phi is not
  * formed.  Storage FP64 or FP32 (one for all
  * runs), n = 6..9.  The geometries must outlive the ensemble. */
+/* ---- HIP graphs of library calls (no counterpart in the reference, which
+ * launches every CuPy kernel from Python each step).  vl_graph_begin starts a
+ * thread-local capture of `stream`; every library call made on that stream
+ * until vl_graph_end is recorded instead of run (calls that synchronise or
+ * allocate fail while capturing: use fixed-dt or device-clock steps, whose
+ * launches neither allocate nor synchronise).  vl_graph_end ends the capture
+ * (always, also when it then reports an error) and instantiates the graph;
+ * vl_graph_launch replays it on a stream; vl_graph_destroy frees it.  The
+ * recorded calls keep the pointers they were given: buffers must outlive the
+ * graph and stay where they are. */
+typedef struct vl_graph *vl_graph_t;
+int vl_graph_begin(void *stream);
+int vl_graph_end(void *stream, vl_graph_t *graph);
+int vl_graph_launch(vl_graph_t graph, void *stream);
+int vl_graph_destroy(vl_graph_t graph);
+
 typedef struct vl_ensemble *vl_ensemble_t;
 int vl_ensemble_create(const vl_geom_t *geoms, int count, vl_ensemble_t *out);
 int vl_ensemble_destroy(vl_ensemble_t e);

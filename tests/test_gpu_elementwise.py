@@ -97,3 +97,49 @@ def test_density_deviation(cuda, sign):
     want = rho.mean() - rho if sign > 0 else rho - rho.mean()
     assert np.abs(out - want).max() < 4e-16
     assert abs(out.sum()) < 1e-13
+
+
+def test_native_graph_replays(cuda):
+    """vl_graph_*: calls recorded on a side stream replay on the current
+    stream; every replay is checked, with eager library work between them."""
+    import torch
+    run, g = _geom()
+    shape = run.grids.shape
+    y = torch.ones(shape, dtype=torch.float64, device=cuda)
+    x = torch.full(shape, 0.5, dtype=torch.float64, device=cuda)
+    side = torch.cuda.Stream(device=cuda)
+    side.wait_stream(torch.cuda.current_stream(cuda))
+    graph = _lib.HipGraph()
+    with graph.capture(side):
+        _lib.axpby(g, y, x, 1.0, 1.0, interior=False)      # y += x, twice
+        _lib.axpby(g, y, x, 1.0, 1.0, interior=False)
+    torch.cuda.current_stream(cuda).wait_stream(side)
+    torch.cuda.synchronize()
+    assert torch.all(y == 1.0)                              # captured, not run
+    want = 1.0
+    for rep in range(3):
+        graph.launch()
+        want += 1.0
+        torch.cuda.synchronize()
+        assert torch.all(y == want), rep
+        _lib.axpby(g, y, None, 2.0, 0.0, interior=False)    # eager work between replays
+        want *= 2.0
+    torch.cuda.synchronize()
+    assert torch.all(y == want)
+
+
+def test_native_graph_capture_error_ends_capture(cuda):
+    """A failing call inside the capture block leaves the stream usable."""
+    import torch
+    run, g = _geom()
+    y = torch.zeros(run.grids.shape, dtype=torch.float64, device=cuda)
+    side = torch.cuda.Stream(device=cuda)
+    graph = _lib.HipGraph()
+    with pytest.raises(_lib.VlasovError):
+        with graph.capture(side):
+            _lib.axpby(g, y, None, 1.0, 1.0)                 # x NULL with b != 0: refused
+    assert graph.handle is None
+    with torch.cuda.stream(side):
+        _lib.axpby(g, y, None, 0.0, 0.0, interior=False)   # the side stream runs eagerly again
+    torch.cuda.synchronize()
+    assert torch.all(y == 0.0)

@@ -28,6 +28,7 @@ EXPORTS = (
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
     "vl_moment_partials", "vl_rk3_step_noalloc", "vl_read_status",
     "vl_axpby", "vl_init_separable", "vl_deviation_to_storage", "vl_total_f64", "vl_density_deviation",
+    "vl_graph_begin", "vl_graph_end", "vl_graph_launch", "vl_graph_destroy",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -110,6 +111,10 @@ def load(path=LIB_PATH):
         "vl_deviation_to_storage": [vp, vp, vp, vp],
         "vl_total_f64": [vp, vp, vp, vp],
         "vl_density_deviation": [vp, vp, vp, i32, vp],
+        "vl_graph_begin": [vp],
+        "vl_graph_end": [vp, ctypes.POINTER(vp)],
+        "vl_graph_launch": [vp, vp],
+        "vl_graph_destroy": [vp],
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],
@@ -203,6 +208,47 @@ def density_deviation(geom, rho, sign=1):
     check(load().vl_density_deviation(geom.handle, ptr(rho), ptr(out), int(sign), stream_ptr()),
           "vl_density_deviation")
     return out
+
+
+class HipGraph:
+    """A HIP graph of library calls (vl_graph_*): ``with g.capture(stream):``
+    records the calls made on ``stream`` (a non-default torch stream) inside
+    the block; ``launch()`` replays them on the current stream."""
+
+    def __init__(self):
+        self.handle = None
+
+    def capture(self, stream):
+        import contextlib
+        import torch
+
+        @contextlib.contextmanager
+        def cm():
+            if self.handle is not None:
+                raise RuntimeError("HipGraph: already captured")
+            h = ctypes.c_void_p()
+            with torch.cuda.stream(stream):
+                check(load().vl_graph_begin(stream.cuda_stream), "vl_graph_begin")
+                try:
+                    yield self
+                except BaseException:
+                    # end the capture (the stream must leave capture mode) and drop the graph
+                    if load().vl_graph_end(stream.cuda_stream, ctypes.byref(h)) == 0 and h.value:
+                        load().vl_graph_destroy(h.value)
+                    raise
+                check(load().vl_graph_end(stream.cuda_stream, ctypes.byref(h)), "vl_graph_end")
+            self.handle = h.value
+        return cm()
+
+    def launch(self, stream=None):
+        if self.handle is None:
+            raise RuntimeError("HipGraph: nothing captured")
+        check(load().vl_graph_launch(self.handle, stream_ptr(stream)), "vl_graph_launch")
+
+    def __del__(self):
+        h, self.handle = self.handle, None
+        if h is not None and _lib is not None:
+            _lib.vl_graph_destroy(h)
 
 
 def stream_ptr(stream=None):

@@ -1383,6 +1383,67 @@ int vl_import_f32(vl_geom_t h, const float *in, void *f, void *stream) {
     return VL_OK;
 }
 
+// ---- HIP graphs of library calls ------------------------------------------
+
+}  // extern "C"
+
+struct vl_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int vl_graph_begin(void *stream) {
+    if (!stream) return fail(VL_E_ARG, "vl_graph_begin: capture needs a stream other than the null stream");
+    VL_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return VL_OK;
+}
+
+int vl_graph_end(void *stream, vl_graph_t *out) {
+    if (!out) return fail(VL_E_NULL, "vl_graph_end: null pointer");
+    *out = nullptr;
+    if (!stream) return fail(VL_E_ARG, "vl_graph_end: null stream");
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e, "vl_graph_end: hipStreamEndCapture");
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(ei, "vl_graph_end: hipGraphInstantiate");
+    }
+    vl_graph *r = new (std::nothrow) vl_graph;
+    if (!r) {
+        (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        return hip_fail(hipErrorOutOfMemory, "vl_graph_end: host allocation");
+    }
+    r->graph = g;
+    r->exec = x;
+    *out = r;
+    return VL_OK;
+}
+
+int vl_graph_launch(vl_graph_t g, void *stream) {
+    if (!g) return fail(VL_E_HANDLE, "vl_graph_launch: null graph");
+    VL_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return VL_OK;
+}
+
+int vl_graph_destroy(vl_graph_t g) {
+    if (!g) return VL_OK;
+    const hipError_t a = hipGraphExecDestroy(g->exec);
+    const hipError_t b = hipGraphDestroy(g->graph);
+    delete g;
+    if (a != hipSuccess) return hip_fail(a, "vl_graph_destroy: hipGraphExecDestroy");
+    if (b != hipSuccess) return hip_fail(b, "vl_graph_destroy: hipGraphDestroy");
+    return VL_OK;
+}
+
 // ---- element-wise passes over whole states ---------------------------------
 
 static PlaneShape plane_shape(const vl_geom *h) {

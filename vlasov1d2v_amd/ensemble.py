@@ -245,12 +245,12 @@ class EnsembleGraph:
         self.energies = torch.zeros((self.steps, len(ens.runs)), dtype=torch.float64, device=dev)
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(torch.cuda.current_stream(dev))
-        self.graph = torch.cuda.CUDAGraph()
+        self.graph = _lib.HipGraph()
         gc.collect()
         gc_was_enabled = gc.isenabled()
         gc.disable()
         try:
-            with torch.cuda.graph(self.graph, stream=side, capture_error_mode="thread_local"):
+            with self.graph.capture(side):
                 for k in range(self.steps):
                     _lib.check(_lib.load().vl_ensemble_rk3_step(ens.handle, _lib.ptr(self.energies[k]),
                                                                 _lib.stream_ptr(side)), "vl_ensemble_rk3_step")
@@ -262,7 +262,7 @@ class EnsembleGraph:
     def replay(self):
         if self.ens.schedule_generation != self.schedule_generation:
             raise _lib.VlasovError("the ensemble schedule changed after this graph was captured: capture again")
-        self.graph.replay()
+        self.graph.launch()
         self.ens.steps_done += self.steps
 
 

@@ -457,14 +457,14 @@ class StepGraph:
         self.clocks = torch.zeros((self.steps, 2), dtype=torch.float64, device=dev)   # (time, dt) per step
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(torch.cuda.current_stream(dev))
-        self.graph = torch.cuda.CUDAGraph()
+        self.graph = _lib.HipGraph()
         # no garbage collection during capture: a collected geometry handle
         # would free device memory in the middle of it
         gc.collect()
         gc_was_enabled = gc.isenabled()
         gc.disable()
         try:
-            with torch.cuda.graph(self.graph, stream=side, capture_error_mode="thread_local"):
+            with self.graph.capture(side):
                 for k in range(self.steps):
                     if adaptive:
                         stepper._fused_step(distribution, elliptic, grids, dg_flux, energy_out=self.energies[k:k + 1],
@@ -489,7 +489,7 @@ class StepGraph:
         if Stepper._token(self._arr, self._ws) != self._mark:
             _lib.check(_lib.load().vl_moment_partials(self._geom.handle, _lib.ptr(self._arr), _lib.ptr(self._ws),
                                                       self._lay, _lib.stream_ptr()), "vl_moment_partials")
-        self.graph.replay()
+        self.graph.launch()
         self._mark = Stepper._token(self._arr, self._ws)
         self.stepper._rho_token = self._mark
         if check:
