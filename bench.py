@@ -125,6 +125,7 @@ def main():
     # input and f^n, write the stage output) -> 3 N s algorithmic bytes each
     launch_bytes = 3 * N * s_bytes
     launch_ms = 0.5 * (head["stage_ms"][1] + head["stage_ms"][2])
+    ceiling = stream_ceiling(N * s_bytes, dev)
     achieved = launch_bytes / (launch_ms / 1e3) / 1e9
     kname = f"k_stage<{w.n},2,{STORAGE_CODES[head['storage']]},{1 if args.layout == 'elem' else 0}>"
     traffic = profiled_traffic(kname, head["storage"])
@@ -146,6 +147,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "stream_ceiling": ceiling, "frac_of_stream_ceiling": achieved / ceiling["achieved"],
                      "kernel": kname + " (fused RHS + stage combination + moment partials; stages 2 and 3)",
                      "launch_ms": launch_ms, "algorithmic_bytes_per_launch": launch_bytes,
                      "stage_ms": head["stage_ms"]},
@@ -164,6 +166,41 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
+
+
+def stream_ceiling(array_bytes, dev, reps=10):
+    """Best rate (GB/s, 3 array_bytes moved per launch) of the streaming probe
+    vl_stream_probe (two non-temporal reads, one non-temporal write, 16-byte
+    vectors) over a few grid sizes, timed with events on the launch stream."""
+    import torch
+    from vlasov1d2v_amd import _lib
+    n = array_bytes // 16 * 2
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.ones(n, dtype=torch.float64, device=dev)
+    out = torch.empty(n, dtype=torch.float64, device=dev)
+    lib = _lib.load()
+    rates = {}
+    for nt in (1, 0):
+        for grid in (4096, 16384, 32768, 65536, 131072):
+            def launch():
+                _lib.check(lib.vl_stream_probe(_lib.ptr(a), _lib.ptr(b), _lib.ptr(out), n * 8, grid, nt,
+                                               _lib.stream_ptr()), "vl_stream_probe")
+            launch()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(reps):
+                launch()
+            t1.record()
+            torch.cuda.synchronize(dev)
+            rates[(nt, grid)] = 3 * n * 8 * reps / (t0.elapsed_time(t1) / 1e3) / 1e9
+    ok = bool(torch.all(out == 2.0).item())
+    del a, b, out
+    torch.cuda.empty_cache()
+    (nt, grid), best = max(rates.items(), key=lambda kv: kv[1])
+    return {"kernel": "k_stream_probe (2 reads + 1 write, 16-byte vectors)", "achieved": best,
+            "unit": "GB/s", "nontemporal": bool(nt), "grid": grid, "array_bytes": n * 8, "checked": ok,
+            "best_nontemporal": max(v for (t, _), v in rates.items() if t),
+            "best_default_policy": max(v for (t, _), v in rates.items() if not t)}
 
 
 def profiled_traffic(kernel, storage):

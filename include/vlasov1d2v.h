@@ -332,6 +332,15 @@ int vl_deviation_to_storage(vl_geom_t h, const double *dev, void *g, void *strea
 int vl_total_f64(vl_geom_t h, const void *f, double *out, void *stream);
 int vl_density_deviation(vl_geom_t h, const double *rho, double *out, int sign, void *stream);
 
+/* Measurement probe, no counterpart in the reference: the stage kernels' HBM
+ * access shape alone -- out = a + b over `bytes` bytes per array (FP64 pairs,
+ * 16-byte vectors; non-temporal loads and stores when `nontemporal`, else the
+ * default cache policy), `grid` workgroups of 256 threads in a grid-stride
+ * loop.  Its best rate (3 bytes moved per array byte) is the practical ceiling
+ * the stage kernels are quoted against. */
+int vl_stream_probe(const void *a, const void *b, void *out, size_t bytes, int grid, int nontemporal,
+                    void *stream);
+
 /* ---- Ensembles (no counterpart in the reference, which steps one run per
  * process through main/main.py:165 -> Stepper.main_loop).  Many independent
  * runs -- each its own geometry (n, grid, physics), state and workspace --

@@ -143,3 +143,18 @@ def test_native_graph_capture_error_ends_capture(cuda):
         _lib.axpby(g, y, None, 0.0, 0.0, interior=False)   # the side stream runs eagerly again
     torch.cuda.synchronize()
     assert torch.all(y == 0.0)
+
+
+@pytest.mark.parametrize("nt", [1, 0])
+@pytest.mark.parametrize("nvals,grid", [(2, 1), (1000, 3), (123456, 64)])
+def test_stream_probe(cuda, nvals, grid, nt):
+    import torch
+    a = torch.arange(nvals, dtype=torch.float64, device=cuda)
+    b = torch.full((nvals,), 0.25, dtype=torch.float64, device=cuda)
+    out = torch.empty_like(a)
+    _lib.check(_lib.load().vl_stream_probe(_lib.ptr(a), _lib.ptr(b), _lib.ptr(out), nvals * 8, grid, nt,
+                                           _lib.stream_ptr()), "vl_stream_probe")
+    assert torch.equal(out, a + 0.25)
+    with pytest.raises(_lib.VlasovError):
+        _lib.check(_lib.load().vl_stream_probe(_lib.ptr(a), _lib.ptr(b), _lib.ptr(out), 8, grid, nt,
+                                               _lib.stream_ptr()), "vl_stream_probe")

@@ -28,7 +28,7 @@ EXPORTS = (
     "vl_poisson_central", "vl_export_f32", "vl_import_f32", "vl_ensemble_set_adaptive", "vl_ensemble_clocks",
     "vl_moment_partials", "vl_rk3_step_noalloc", "vl_read_status",
     "vl_axpby", "vl_init_separable", "vl_deviation_to_storage", "vl_total_f64", "vl_density_deviation",
-    "vl_graph_begin", "vl_graph_end", "vl_graph_launch", "vl_graph_destroy",
+    "vl_graph_begin", "vl_graph_end", "vl_graph_launch", "vl_graph_destroy", "vl_stream_probe",
 )
 VL_TUNE_XSEG = 1
 VL_TUNE_MFMA = 2
@@ -115,6 +115,7 @@ def load(path=LIB_PATH):
         "vl_graph_end": [vp, ctypes.POINTER(vp)],
         "vl_graph_launch": [vp, vp],
         "vl_graph_destroy": [vp],
+        "vl_stream_probe": [vp, vp, vp, sz, i32, i32, vp],
         "vl_set_cfl": [vp, f64, f64, f64, f64],
         "vl_clock_init": [vp, vp, f64, vp, vp],
         "vl_rk3_step_adaptive": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp],

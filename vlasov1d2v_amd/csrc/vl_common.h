@@ -39,6 +39,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int64
 }
 typedef unsigned int vl_u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int vl_u32x4 __attribute__((ext_vector_type(4)));
+typedef double vl_f64x2 __attribute__((ext_vector_type(2)));
 
 // Component k (a compile-time constant after unrolling) of a 16-byte vector.
 // Named components, not v[k]: with the subscript form hipcc has read lane 0

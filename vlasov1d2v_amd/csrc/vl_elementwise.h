@@ -109,4 +109,40 @@ __global__ __launch_bounds__(kBlock) void k_density_deviation(const double *__re
     for (int i = t; i < count; i += kBlock) out[i] = sign > 0 ? mean - rho[i] : rho[i] - mean;
 }
 
+// Streaming probe of the stage kernels' HBM access shape: two arrays read
+// once and one written once, 16-byte vectors, with non-temporal loads and
+// stores (NT, the policy of the stage output and f^n) or the default policy.
+// out = a + b over doubles; a grid-stride loop with four vectors in flight per
+// thread.  Its best rate is the practical ceiling the stage kernels are
+// quoted against, beside the 8 TB/s peak.
+template <bool NT>
+__device__ __forceinline__ vl_f64x2 probe_ld(const vl_f64x2 *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void probe_st(vl_f64x2 v, vl_f64x2 *p) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <bool NT>
+__global__ __launch_bounds__(kBlock) void k_stream_probe(const vl_f64x2 *__restrict__ a,
+                                                         const vl_f64x2 *__restrict__ b,
+                                                         vl_f64x2 *__restrict__ out, int64_t nvec) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 3 * stride < nvec; i += 4 * stride) {
+        vl_f64x2 x[4], y[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = probe_ld<NT>(a + i + k * stride);
+            y[k] = probe_ld<NT>(b + i + k * stride);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) probe_st<NT>(x[k] + y[k], out + i + k * stride);
+    }
+    for (; i < nvec; i += stride) probe_st<NT>(probe_ld<NT>(a + i) + probe_ld<NT>(b + i), out + i);
+}
+
 }  // namespace vl

@@ -1537,6 +1537,24 @@ int vl_density_deviation(vl_geom_t h, const double *rho, double *out, int sign, 
     return VL_OK;
 }
 
+int vl_stream_probe(const void *a, const void *b, void *out, size_t bytes, int grid, int nontemporal,
+                    void *stream) {
+    if (!a || !b || !out) return fail(VL_E_NULL, "vl_stream_probe: null pointer");
+    if (!aligned(a, 16) || !aligned(b, 16) || !aligned(out, 16) || bytes % 16 != 0)
+        return fail(VL_E_ALIGN, "vl_stream_probe: 16-byte aligned arrays of whole 16-byte vectors required");
+    if (grid <= 0 || grid > (1 << 20)) return fail(VL_E_ARG, "vl_stream_probe: grid %d outside [1, 2^20]", grid);
+    const int64_t nvec = (int64_t)(bytes / 16);
+    if (nvec == 0) return VL_OK;
+    if (nontemporal)
+        hipLaunchKernelGGL(k_stream_probe<true>, dim3((unsigned)grid), dim3(kBlock), 0, (hipStream_t)stream,
+                           (const vl_f64x2 *)a, (const vl_f64x2 *)b, (vl_f64x2 *)out, nvec);
+    else
+        hipLaunchKernelGGL(k_stream_probe<false>, dim3((unsigned)grid), dim3(kBlock), 0, (hipStream_t)stream,
+                           (const vl_f64x2 *)a, (const vl_f64x2 *)b, (vl_f64x2 *)out, nvec);
+    VL_HIP(hipGetLastError());
+    return VL_OK;
+}
+
 int vl_total_f64(vl_geom_t h, const void *f, double *out, void *stream) {
     if (int r = record_check(h, f, out, "vl_total_f64")) return r;
     if (!aligned(f, storage_bytes(h->storage)) || !aligned(out, 8)) return fail(VL_E_ALIGN, "vl_total_f64: misaligned");
