@@ -152,19 +152,58 @@ template <> struct RsPack<3> {              // FP16
     }
 };
 
+// Marsaglia's xorshift32 (13, 17, 5): the random words of stochastic
+// rounding within one lane's share of a tile.  Each lane starts from a hash of
+// (stored element, lane), so the words depend on where a value is stored and
+// on the seed, not on the launch shape; full-rate shifts and xors only.
+__device__ __forceinline__ unsigned xorshift32(unsigned x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Vector-memory loads one lane issues for the next tile's prefetch: per own
-// row its 16 bytes, the two v-face words and (n = 9) the tail word, then the
-// neighbour row and (n = 9) its tail.  Every load is issued, masked lanes with
-// an out-of-range offset, so the count is fixed.
+// Vector-memory loads a wave issues for the next tile's prefetch, at least:
+// per own row its 16 bytes, the two v-face words and (n = 9) the tail word,
+// then the neighbour row and (n = 9) its tail.  These are issued whatever the
+// lanes (masked lanes with an out-of-range offset); pencils at the u boundary
+// issue the neighbour loads once more (ghost elements), so a wait for all but
+// this many loads may wait a little longer there, never shorter.
 template <int N>
 constexpr int rs_prefetch_loads() {
     return (N == 9 ? 4 : 3) * ((N * N + 63) / 64) + (N == 9 ? 2 : 1);
+}
+
+// f^n staging into LDS: buffer loads with the LDS flag (lane l's 16 or 4
+// bytes land at lds + 16 l or 4 l), written as inline asm.  The same loads
+// through the builtin make the compiler wait for every outstanding load at
+// the next LDS access of any array (it cannot tell the staging buffer from
+// the others), which would wait for the next tile's prefetch at once; here
+// the kernel waits for them itself (rs_wait_vm) before it reads the staging
+// buffer.  The asm sets M0 (the LDS base of the wave) and reads no VGPR but
+// the offset, so no register of it is written while the load is in flight.
+__device__ __forceinline__ vl_u32x4 rsrc_words(const void *p, int bytes) {
+    const unsigned long long a = (unsigned long long)p;
+    return vl_u32x4{(unsigned)a, (unsigned)(a >> 32) & 0xffffu, (unsigned)bytes, 0x00020000u};   // as make_rsrc
+}
+__device__ __forceinline__ unsigned lds_addr(const void *p) {
+    return (unsigned)(unsigned long long)(const __attribute__((address_space(3))) void *)p;
+}
+__device__ __forceinline__ void lds_dma16_nt(const vl_u32x4 &r, int voff, unsigned lds) {
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tbuffer_load_dwordx4 %0, %1, 0 offen nt lds" ::"v"(voff), "s"(r),
+                 "s"(lds)
+                 : "memory", "m0");
+}
+__device__ __forceinline__ void lds_dma4_nt(const vl_u32x4 &r, int voff, unsigned lds) {
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tbuffer_load_dword %0, %1, 0 offen nt lds" ::"v"(voff), "s"(r),
+                 "s"(lds)
+                 : "memory", "m0");
 }
 
 // s_waitcnt through the builtin, so that the compiler's own wait insertion
@@ -302,7 +341,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const int sbytes = (int)((int64_t)(s.nx + 2) * sxe * 2);
     const __amdgpu_buffer_rsrc_t rin = make_rsrc(fin, sbytes);
     const __amdgpu_buffer_rsrc_t rgh = make_rsrc(fgh, sbytes);
-    const __amdgpu_buffer_rsrc_t rfn = make_rsrc(fn, sbytes);
+    const vl_u32x4 rfnw = rsrc_words(fn, sbytes);
     const __amdgpu_buffer_rsrc_t rout = make_rsrc(fout, sbytes);
     auto ld16 = [&](__amdgpu_buffer_rsrc_t rr, int off) {
         return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(rr, off, 0, 0);
@@ -317,8 +356,9 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const int nside = ln / N, nj = ln - (ln / N) * N;
     const bool nlane = ln < NNB;
     const int nrr = nside == 0 ? (N - 1) * N + nj : nside == 1 ? nj : nside == 2 ? nj * N + (N - 1) : nj * N;
-    const __amdgpu_buffer_rsrc_t ruL = eu - 1 == 0 ? rgh : rin;        // wave-uniform
-    const __amdgpu_buffer_rsrc_t ruR = eu + 1 == s.nu + 1 ? rgh : rin;
+    // neighbour rows of the u ghost elements (first / last u element) come
+    // from fgh; lanes whose neighbour is one of them (wave-uniform per side)
+    const bool nbr_gh = (nside == 2 && eu - 1 == 0) || (nside == 3 && eu + 1 == s.nu + 1);
     auto load = [&](int ex, RsLoads<N> &d) {
         if (!active) return;
         const int b = (int)(ebase(ex, eu, ev) * 2);
@@ -334,21 +374,23 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             d.vnl[k] = ld32(rL, ok ? (bl + (N == 9 ? (8 * ROWS + r) * 2 : r * 16 + 12)) & ~3 : kOob);
             d.vnr[k] = ld32(rR, ok ? br + r * 16 : kOob);
         }
-        const int exn = nside == 0 ? xmap(ex - 1) : nside == 1 ? xmap(ex + 1) : ex;
-        const int eun = nside == 2 ? eu - 1 : nside == 3 ? eu + 1 : eu;
-        const int bn = (int)(ebase(exn, eun, ev) * 2);
+        // the neighbour element of each side is uniform: four scalar bases and
+        // a per-lane select; the neighbour rows come from fin, except on the
+        // first / last u element, whose u neighbours are ghost elements read
+        // from fgh by a second load of just those lanes (same registers)
+        const int xl = ex - 1 == 0 ? s.nx : ex - 1, xr = ex + 1 == s.nx + 1 ? 1 : ex + 1;   // periodic x
+        const int bx0 = (int)(ebase(xl, eu, ev) * 2), bx1 = (int)(ebase(xr, eu, ev) * 2);
+        const int bu0 = (int)(ebase(ex, eu - 1, ev) * 2), bu1 = (int)(ebase(ex, eu + 1, ev) * 2);
+        int bn = bu1;
+        bn = nside == 2 ? bu0 : bn;
+        bn = nside == 1 ? bx1 : bn;
+        bn = nside == 0 ? bx0 : bn;
         const int orow = nlane ? bn + nrr * 16 : kOob, otail = nlane ? (bn + (8 * ROWS + nrr) * 2) & ~3 : kOob;
-        // the resource is uniform per side: u-neighbours of the first / last
-        // u-element are ghost elements, read from fgh
-        if (nside == 2) {
-            d.nrow = buf_ld16(ruL, orow);
-            d.ntail = N == 9 ? ld32(ruL, otail) : 0u;
-        } else if (nside == 3) {
-            d.nrow = buf_ld16(ruR, orow);
-            d.ntail = N == 9 ? ld32(ruR, otail) : 0u;
-        } else {
-            d.nrow = buf_ld16(rin, orow);
-            d.ntail = N == 9 ? ld32(rin, otail) : 0u;
+        d.nrow = buf_ld16(rin, orow);
+        d.ntail = N == 9 ? ld32(rin, otail) : 0u;
+        if (nbr_gh) {
+            d.nrow = buf_ld16(rgh, orow);
+            if (N == 9) d.ntail = ld32(rgh, otail);
         }
     };
     // registers -> brick (slots 0..11 of every row that the passes read)
@@ -398,34 +440,28 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
 
     // Per tile: fill the brick from `cur`, prefetch the next tile into `nxt`,
     // compute, store.
-    // moment weight of each own row's u-node (rows ln + 64 k)
-    float wuk[NRI];
-#pragma unroll
-    for (int k = 0; k < NRI; ++k) {
-        const int r = ln + 64 * k;
-        wuk[k] = r < ROWS ? ((const float *)tabp->wu)[r - (r / N) * N] : 0.f;
-    }
     rs_wait_all();
 
-    RsLoads<N> cur, nxt;
-    load(pc.ex_begin, cur);
+    // The brick of a tile is filled at the end of the previous tile, after
+    // its epilogue: the prefetched loads are then older than that epilogue's
+    // stores, which stay in flight (and the compiler's waits see one path).
+    RsLoads<N> nxt;
+    {
+        RsLoads<N> first;
+        load(pc.ex_begin, first);
+        rs_wait_all();
+        fill(first);
+    }
     for (int ex = pc.ex_begin; ex <= pc.ex_end; ++ex) {
         const bool has_next = ex < pc.ex_end;
-        fill(cur);
         if (MODE == kModeStage && active) {
             // f^n rows of this element straight into LDS: lane l -> row l,
             // then rows 64.. (lanes < ROWS - 64); n = 9: the tail dwords
             const int bfn = (int)(ebase(ex, eu, ev) * 2);
-            unsigned *st0 = fnst + el * FNW;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)st0, 16,
-                                                     bfn + ln * 16, 0, 0, kCachePolicyNT);
-            if (ln < ROWS - 64)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + 256),
-                                                         16, bfn + (64 + ln) * 16, 0, 0, kCachePolicyNT);
-            if (N == 9)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rfn, (__attribute__((address_space(3))) void *)(st0 + FNR),
-                                                         4, ln < (ROWS + 1) / 2 ? bfn + 16 * ROWS + 4 * ln : kOob,
-                                                         0, 0, kCachePolicyNT);
+            const unsigned st0 = lds_addr(fnst + el * FNW);
+            lds_dma16_nt(rfnw, bfn + ln * 16, st0);
+            if (ln < ROWS - 64) lds_dma16_nt(rfnw, bfn + (64 + ln) * 16, st0 + 256 * 4);
+            if (N == 9) lds_dma4_nt(rfnw, ln < (ROWS + 1) / 2 ? bfn + 16 * ROWS + 4 * ln : kOob, st0 + FNR * 4);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (has_next) load(ex + 1, nxt);
@@ -490,6 +526,11 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             wave_sync_lds();
             // ---- epilogue: whole rows ----------------------------------------
             const int64_t bo = ebase(ex, eu, ev);
+            // stochastic rounding: this lane's word stream for this element
+            unsigned rstate = sround ? (hash32(((unsigned)((ex * (s.nu + 2) + eu) * vrow + ev) << 6 | (unsigned)ln) ^
+                                              seedh) |
+                                       1u)
+                                     : 0u;
             if (MODE == kModeStage) {
                 // the staged f^n is in LDS: its loads were issued before the next tile's
                 // prefetch, which may stay in flight
@@ -498,10 +539,15 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                 else
                     rs_wait_vm<0>();
             }
+            // Branch-free passes: lanes past the last row recompute it and their
+            // stores go to an out-of-range offset (dropped), so every pass issues
+            // the same stores (the next tile's wait for its prefetch counts them).
 #pragma unroll 1
             for (int k = 0; k < NRI; ++k) {
-                const int r = ln + 64 * k;
-                if (r < ROWS) {
+                const int r0 = ln + 64 * k;
+                const bool rok = r0 < ROWS;
+                const int r = rok ? r0 : ROWS - 1;
+                {
                     vl_u32x4 fnr = {0u, 0u, 0u, 0u};
                     unsigned fnt = 0u;
                     if (MODE == kModeStage) {
@@ -516,8 +562,6 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                     const unsigned nw[4] = {fnr.x, fnr.y, fnr.z, fnr.w};
                     ConstOpTables &tab = *(ConstOpTables *)tabp;
                     // pairs of values: combine, round, moment of the stored pair
-                    // random words of this row (stochastic rounding): hash of (value row, pair)
-                    const unsigned rowgid = ((unsigned)(((ex * (s.nu + 2) + eu) * vrow + ev) * ROWS + r)) << 3;
                     auto pair = [&](float l0, float l1, unsigned fiw, unsigned fnw, float &m, int j) {
                         float o0, o1;
                         if (MODE == kModeStage1) {
@@ -527,8 +571,13 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                             o0 = (c0 * PK::lo(fnw) + c1 * PK::lo(fiw)) + c2 * l0;
                             o1 = (c0 * PK::hi(fnw) + c1 * PK::hi(fiw)) + c2 * l1;
                         }
-                        const unsigned wv2 = sround ? PK::sr2(o0, o1, hash32((rowgid | (unsigned)(j >> 1)) ^ seedh))
-                                                    : F::enc2(o0, o1);
+                        unsigned wv2;
+                        if (sround) {
+                            rstate = xorshift32(rstate);
+                            wv2 = PK::sr2(o0, o1, rstate);
+                        } else {
+                            wv2 = F::enc2(o0, o1);
+                        }
                         m += tab.wv[j] * PK::lo(wv2);
                         m += tab.wv[j + 1] * PK::hi(wv2);
                         return wv2;
@@ -547,22 +596,25 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                         w.z = pair(l1[0], l1[1], f23.x, nw[2], m, 4);
                         w.w = pair(l1[2], l1[3], f23.y, nw[3], m, 6);
                     }
-                    buf_st16_nt(rout, (int)(bo * 2) + r * 16, w);
+                    buf_st16_nt(rout, rok ? (int)(bo * 2) + r * 16 : kOob, w);
                     if (N == 9) {
                         const float fi8 = PK::lo(br[4]);
                         const float fn8 = (r & 1) ? PK::hi(fnt) : PK::lo(fnt);
                         const float o8 = MODE == kModeStage1 ? fi8 + c2 * xp[8] : (c0 * fn8 + c1 * fi8) + c2 * xp[8];
-                        const unsigned t8 = sround ? PK::sr1(o8, hash32((rowgid | 4u) ^ seedh)) : F::enc(o8);
-                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)t8, rout, (int)(bo * 2) + (8 * ROWS + r) * 2,
+                        unsigned t8;
+                        if (sround) {
+                            rstate = xorshift32(rstate);
+                            t8 = PK::sr1(o8, rstate);
+                        } else {
+                            t8 = F::enc(o8);
+                        }
+                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)t8, rout,
+                                                              rok ? (int)(bo * 2) + (8 * ROWS + r) * 2 : kOob,
                                                               0, 0);
                         m += tab.wv[8] * PK::lo(t8);
                     }
-                    // moment partial of the stored values (main/grid.py:319-321)
-                    float wr = wuk[0];
-#pragma unroll
-                    for (int q = 1; q < NRI; ++q)
-                        if (k == q) wr = wuk[q];
-                    red[el * ROWS + r] = wr * m;
+                    // v-moment of the stored row (its u weight is applied in the sum below)
+                    if (rok) red[el * ROWS + r] = m;
                 }
             }
         }
@@ -572,14 +624,17 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             wave_sync_lds();
             if (ln < N) {
                 double a = 0.0;
-                if (active)
-                    for (int iu = 0; iu < N; ++iu) a += (double)red[el * ROWS + ln * N + iu];
+                if (active) {
+                    ConstOpTables &tab = *(ConstOpTables *)tabp;
+#pragma unroll
+                    for (int iu = 0; iu < N; ++iu) a += (double)(tab.wu[iu] * red[el * ROWS + ln * N + iu]);
+                }
                 const int64_t slot =
                     (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + ln;
                 s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
             }
         }
-        cur = nxt;
+        if (has_next) fill(nxt);
     }
 }
 

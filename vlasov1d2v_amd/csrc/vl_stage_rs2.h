@@ -92,13 +92,6 @@ __device__ __forceinline__ vl_u32x2 lds_tr16(const unsigned *p) {
     return __builtin_bit_cast(vl_u32x2, r);
 }
 
-__device__ __forceinline__ unsigned xorshift32(unsigned x) {
-    x ^= x << 13;
-    x ^= x >> 17;
-    x ^= x << 5;
-    return x;
-}
-
 template <int N, int MODE, int S, bool ADT = false>
 __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
     const uint16_t *__restrict__ fin,   // stage input
@@ -271,8 +264,7 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
     const int nside = ln / N, nj = ln - (ln / N) * N;
     const bool nlane = ln < NNB;
     const int nrr = nside == 0 ? (N - 1) * N + nj : nside == 1 ? nj : nside == 2 ? nj * N + (N - 1) : nj * N;
-    const __amdgpu_buffer_rsrc_t ruL = eu - 1 == 0 ? rgh : rin;
-    const __amdgpu_buffer_rsrc_t ruR = eu + 1 == s.nu + 1 ? rgh : rin;
+    const bool nbr_gh = (nside == 2 && eu - 1 == 0) || (nside == 3 && eu + 1 == s.nu + 1);
     auto load = [&](int ex, RsLoads<N> &d) {
         if (!active) return;
         const int b = (int)(ebase(ex, eu, ev) * 2);
@@ -288,19 +280,23 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
             d.vnl[k] = ld32(rL, ok ? (bl + (8 * ROWS + r) * 2) & ~3 : kOob);
             d.vnr[k] = ld32(rR, ok ? br + r * 16 : kOob);
         }
-        const int exn = nside == 0 ? xmap(ex - 1) : nside == 1 ? xmap(ex + 1) : ex;
-        const int eun = nside == 2 ? eu - 1 : nside == 3 ? eu + 1 : eu;
-        const int bn = (int)(ebase(exn, eun, ev) * 2);
+        // the neighbour element of each side is uniform: four scalar bases and
+        // a per-lane select; the neighbour rows come from fin, except on the
+        // first / last u element, whose u neighbours are ghost elements read
+        // from fgh by a second load of just those lanes (same registers)
+        const int xl = ex - 1 == 0 ? s.nx : ex - 1, xr = ex + 1 == s.nx + 1 ? 1 : ex + 1;   // periodic x
+        const int bx0 = (int)(ebase(xl, eu, ev) * 2), bx1 = (int)(ebase(xr, eu, ev) * 2);
+        const int bu0 = (int)(ebase(ex, eu - 1, ev) * 2), bu1 = (int)(ebase(ex, eu + 1, ev) * 2);
+        int bn = bu1;
+        bn = nside == 2 ? bu0 : bn;
+        bn = nside == 1 ? bx1 : bn;
+        bn = nside == 0 ? bx0 : bn;
         const int orow = nlane ? bn + nrr * 16 : kOob, otail = nlane ? (bn + (8 * ROWS + nrr) * 2) & ~3 : kOob;
-        if (nside == 2) {
-            d.nrow = buf_ld16(ruL, orow);
-            d.ntail = ld32(ruL, otail);
-        } else if (nside == 3) {
-            d.nrow = buf_ld16(ruR, orow);
-            d.ntail = ld32(ruR, otail);
-        } else {
-            d.nrow = buf_ld16(rin, orow);
-            d.ntail = ld32(rin, otail);
+        d.nrow = buf_ld16(rin, orow);
+        d.ntail = N == 9 ? ld32(rin, otail) : 0u;
+        if (nbr_gh) {
+            d.nrow = buf_ld16(rgh, orow);
+            if (N == 9) d.ntail = ld32(rgh, otail);
         }
     };
     // registers -> brick
