@@ -127,7 +127,11 @@ def main():
     launch_ms = 0.5 * (head["stage_ms"][1] + head["stage_ms"][2])
     ceiling = stream_ceiling(N * s_bytes, dev)
     achieved = launch_bytes / (launch_ms / 1e3) / 1e9
-    kname = f"k_stage<{w.n},2,{STORAGE_CODES[head['storage']]},{1 if args.layout == 'elem' else 0}>"
+    code = STORAGE_CODES[head["storage"]]
+    if head["storage"] in ("bf16", "f16") and w.n in (8, 9) and args.layout == "elem":
+        kname = f"k_stage_rs<{w.n},2,{code}>"        # the row-split matrix-core kernel
+    else:
+        kname = f"k_stage<{w.n},2,{code},{1 if args.layout == 'elem' else 0}>"
     traffic = profiled_traffic(kname, head["storage"])
     out = {
         "metric": "RK3 DOF-updates/s at 116.6M DOF, FP64 vs BF16-store; % HBM roofline",
