@@ -31,14 +31,14 @@
 // one (x, u) row.
 //   v-direction: the rows of a fixed x-slot are the B operand of the MFMA
 //     directly (columns = u-slots, K = v-slots).
-//   u-direction: the same registers multiplied by the identity matrix give
-//     the transposed 16 x 16 block (lane = v-slot, K = u-slots), exact; the
-//     line operator then runs with the data as the A operand.  Its output
-//     has the layout of the v-direction's output (lane = u-node, four
-//     v-nodes), so the two add in registers.
-//   x-direction: same transposition of a fixed-u-slot block (rows = x-slots);
-//     results go to an FP32 LDS array X[ix][iu][v-slot] with one 16-byte
-//     store per lane, and the u/v pass adds them.
+//   u-direction: a transposing LDS read (ds_read_b64_tr_b16) of the same
+//     x-slot's rows gives the 16 x 16 block with lane = v-slot, K = u-slots;
+//     the line operator then runs with the data as the A operand.  Its
+//     output has the layout of the v-direction's output (lane = u-node,
+//     four v-nodes), so the two add in registers.
+//   x-direction: the same transposing read of a fixed-u-slot block (rows =
+//     x-slots); results go to an FP32 LDS array X[ix][iu][v-slot] with one
+//     16-byte store per lane, and the u/v pass adds them.
 // The speeds multiply the MFMA results (they are constant along each line);
 // the background images of the reduced forms, scale * A+-_u f0 and
 // scale * A_v f0, are the accumulator inputs of those MFMAs.
@@ -151,6 +151,21 @@ template <> struct RsPack<3> {              // FP16
         return __builtin_bit_cast(unsigned, v) & 0xffffu;
     }
 };
+
+typedef short vl_s16x4b __attribute__((ext_vector_type(4)));
+
+// gfx950 LDS transpose read (ds_read_b64_tr_b16), lane layout probed on
+// MI355X: in each 16-lane group, lane j = 4 a + b receives element b of the
+// 8-byte chunks addressed by lanes a, a + 4, a + 8 and a + 12 of its group.
+// With lane c = a + 4 t pointing at (slot 4 g + t, v-quad a) of brick lines
+// (g = lane group), lane j holds slots 4 g .. 4 g + 3 of v-slot 4 a + b:
+// a matrix-core A operand with lines = v-slots and K = slots, read straight
+// from LDS (no identity product and repacking).
+__device__ __forceinline__ vl_u32x2 lds_tr16(const unsigned *p) {
+    const vl_s16x4b r = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) vl_s16x4b *)(__attribute__((address_space(3))) void *)p);
+    return __builtin_bit_cast(vl_u32x2, r);
+}
 
 // Marsaglia's xorshift32 (13, 17, 5): the random words of stochastic
 // rounding within one lane's share of a tile.  Each lane starts from a hash of
@@ -285,16 +300,19 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     // ---- per-wave constants ------------------------------------------------
     // operator fragments (host-built): lane holds op[i = lj][slot 4 lg + t],
     // hi and lo 16-bit parts
-    vl_u32x2 ophi, oplo, idv;
+    vl_u32x2 ophi, oplo;
     {
         const vl_u32x4 fr = *reinterpret_cast<const vl_u32x4 *>(s.mxfrag + ((S == 2 ? 0 : 64) + ln) * 4);
         ophi = vl_u32x2{fr.x, fr.y};
         oplo = vl_u32x2{fr.z, fr.w};
-        const unsigned one = F::enc(1.0f);
-        const int d = lj - 4 * lg;   // identity: 1 at slot lj
-        idv = vl_u32x2{(d == 0 ? one : 0u) | ((d == 1 ? one : 0u) << 16),
-                       (d == 2 ? one : 0u) | ((d == 3 ? one : 0u) << 16)};
     }
+    // transposing reads of the x- and u-direction operands (lds_tr16): this
+    // lane points at slot 4 lg + t (clamped to 11: slots 12..15 meet zero
+    // operator columns) and v-quad a (quad 3 repeats quad 2: lines 12..15 are
+    // padding); the block adds su RSD (x pass) or sx PSD (u pass)
+    const int tr_k = min(4 * lg + (lj >> 2), 11), tr_q = 2 * min(lj & 3, 2);
+    const int tr_x = tr_k * PSD + tr_q;     // x-slot tr_k of u-slot su
+    const int tr_u = tr_k * RSD + tr_q;     // u-slot tr_k of x-slot sx
     // background images (accumulator inputs), v-speed terms: lane column
     // iu = lj, rows iv = 4 lg + t of this wave's element
     // u background as average + half difference: su * G(sign su) = su Gavg + |su| Gh
@@ -472,7 +490,6 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             // (finite) brick row -- it only meets zero operator columns or
             // feeds output lines nobody keeps -- and lanes without an output
             // node store to their sink.
-            const int rdx = min(lj, 11) * PSD + 2 * min(lg, 2);    // x-slot lj, v-slots 4 lg ..
             const int rduv = min(lj, 11) * RSD + 2 * min(lg, 2);   // u-slot lj, v-slots 4 lg ..
             const bool wr = lj < N && lg <= GMAX;
             float *const snk = sink + ln * 4;
@@ -480,9 +497,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             // ---- x: X[ix][iu][v-slot] = A_x g (the x speed is applied in the u/v pass)
 #pragma unroll
             for (int su = 0; su < N; ++su) {
-                const vl_u32x2 r = *reinterpret_cast<const vl_u32x2 *>(bk + rdx + su * RSD);
-                const vl_f32x4 tr = F::mfma(r, idv, zero4);   // lane: v-slot lj, x-slots 4 lg + t
-                vl_u32x2 p = {PK::exact2(tr[0], tr[1]), PK::exact2(tr[2], tr[3])};
+                vl_u32x2 p = lds_tr16(bk + tr_x + su * RSD);   // lane: v-slot lj, x-slots 4 lg + t
                 p.y = __builtin_amdgcn_perm(p.x, p.y, ((xposm >> su) & 1u) ? selp : seln);
                 vl_f32x4 d = F::mfma(p, ophi, zero4);
                 d = F::mfma(p, oplo, d);                      // lane: x-node lj, v-slots 4 lg + t
@@ -506,8 +521,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                 bvv.y = __builtin_amdgcn_perm(r.x, r.y, vsel);
                 vl_f32x4 dv = F::mfma(ophi, bvv, hv);
                 dv = F::mfma(oplo, bvv, dv);                  // lane: u-slot lj, v-nodes 4 lg + t
-                const vl_f32x4 tr = F::mfma(r, idv, zero4);   // lane: v-slot lj, u-slots 4 lg + t
-                vl_u32x2 p = {PK::exact2(tr[0], tr[1]), PK::exact2(tr[2], tr[3])};
+                vl_u32x2 p = lds_tr16(bk + tr_u + sx * PSD);   // lane: v-slot lj, u-slots 4 lg + t
                 p.y = __builtin_amdgcn_perm(p.x, p.y, (q + pl >= 0.f) ? selp : seln);   // Ju a_u of the lane's line
                 vl_f32x4 du = F::mfma(p, ophi, gavg);
                 du = F::mfma(p, oplo, du);                    // lane: u-node lj, v-slots 4 lg + t

@@ -78,19 +78,10 @@ __device__ __forceinline__ int rs2_row(int xs, int us) {
     return SH::UEXT + SH::RW * (SH::EX * xs + (us - N));
 }
 
-typedef short vl_s16x4b __attribute__((ext_vector_type(4)));
-
 // Waves per SIMD the natural-frame kernel is compiled for (register budget).
 #ifndef VL_RS2_WAVES
 #define VL_RS2_WAVES 3
 #endif
-
-// gfx950 LDS transpose read of one 16-bit B fragment (see the header).
-__device__ __forceinline__ vl_u32x2 lds_tr16(const unsigned *p) {
-    const vl_s16x4b r = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-        (__attribute__((address_space(3))) vl_s16x4b *)(__attribute__((address_space(3))) void *)p);
-    return __builtin_bit_cast(vl_u32x2, r);
-}
 
 template <int N, int MODE, int S, bool ADT = false>
 __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
