@@ -48,10 +48,17 @@
 // independent (no barriers): each writes the moment partial of its own
 // element, one slot per tile element.
 // The global loads of the next tile are in flight while the current one is
-// computed; the epilogue works on whole rows (16-byte loads and stores).
-// Stages 2/3 read f^n of the current element with direct-to-LDS buffer
-// loads issued at the start of the tile (no registers held across the
-// passes); the epilogue waits for them and reads the rows from LDS.
+// computed.  Stages 2/3 read f^n of the current element with direct-to-LDS
+// buffer loads issued at the start of the tile (no registers held across
+// the passes); the epilogue waits for them and reads them from LDS.
+//
+// Epilogue.  The element's 729 values are taken as units of four values
+// that are consecutive in the storage order: 162 half rows, then (n = 9) the
+// tail block in 21 groups of four rows.  A wave does one unit per lane and
+// pass, 3 passes (n = 9; 2 for n = 8): the stage combination, the rounding,
+// an 8-byte store and the w_v-weighted moment of the stored values.  The
+// per-x-node moment is then summed over the unit results by four lanes per
+// node (w_u applied there) and a quad reduction.
 #pragma once
 #include "vl_stage_mx.h"
 
@@ -84,10 +91,27 @@ struct RsShape {
     static constexpr int RSD = 6;                 // brick row stride (dwords, 12 slots)
     static constexpr int PSD = 74;                // brick x-slot stride (dwords): 12 rows + 2
     static constexpr int ESD = 12 * PSD;          // brick of one element (dwords)
-    static constexpr int XR = 12;                 // floats per X row (v-slots 0..11)
-    static constexpr int XE = ROWS * XR;          // floats of X per element
     static constexpr int GMAX = N == 9 ? 2 : 1;   // last lane group holding an output node
     static constexpr int NNB = 4 * N;             // neighbour rows per tile element
+    // Epilogue units: four values that are consecutive in the element's
+    // storage order -- half rows (row r, nodes 4q .. 4q+3: unit 2r + q),
+    // then (n = 9) the tail block in groups of four rows (unit 2 n^2 + m:
+    // node 8 of rows 4m .. 4m+3).  Unit u holds storage values 4u .. 4u+3.
+    static constexpr int UH = 2 * ROWS;                  // half-row units
+    static constexpr int UT = N == 9 ? (ROWS + 3) / 4 : 0;   // tail units
+    static constexpr int NUNIT = UH + UT;                // 183 (n = 9), 128 (n = 8)
+    static constexpr int NPASS = (NUNIT + 63) / 64;      // epilogue passes of one unit per lane
+    static constexpr int XR = 12;                        // floats per L row (v-slots 0..11)
+    static constexpr int XE = ROWS * XR;                 // floats of L per element
+    static constexpr int TAIL = 8 * ROWS;                // storage index of the tail block
+    static constexpr int TF = N == 9 ? 4 * UT : 4;       // tail values of the stage input (16-bit)
+    // per-wave scratch (floats): 16 bytes per lane as the sink of the passes'
+    // lanes that hold no output node, then the epilogue's moment sums:
+    // half-row units at [0, UH), the weighted tail values of row r at TWO + r
+    static constexpr int RW = 256;
+    static constexpr int TWO = (UH + 3) / 4 * 4;
+    static constexpr int RDUMMY = RW - 8;                // discarded moment writes
+    static_assert(N == 8 || TWO + 4 * UT <= RDUMMY, "moment scratch layout");
 };
 // 8-byte reads at row stride 6 dwords and x-slot stride 74 dwords hit 24 distinct
 // bank pairs for the 12 rows (or x-slots) a lane group reads.
@@ -117,7 +141,8 @@ template <> struct RsPack<2> {              // BF16
     // gfx950's v_cvt_sr_bf16_f32 does exactly that with the HIGH 16 bits of
     // its random operand: a takes the low half of r, b the high half.
     __device__ static unsigned sr2(float a, float b, unsigned r) {
-        vl_bf16x2 v = {};
+        // both halves are written: the initial value is irrelevant
+        vl_bf16x2 v = __builtin_bit_cast(vl_bf16x2, r);
         v = __builtin_amdgcn_cvt_sr_bf16_f32(v, a, r << 16, false);
         v = __builtin_amdgcn_cvt_sr_bf16_f32(v, b, r, true);
         return __builtin_bit_cast(unsigned, v);
@@ -140,7 +165,7 @@ template <> struct RsPack<3> {              // FP16
     // discarded fraction; also unbiased in the subnormal range).  a takes
     // bits 31..19 of r, b bits 18..6.
     __device__ static unsigned sr2(float a, float b, unsigned r) {
-        vl_f16x2 v = {};
+        vl_f16x2 v = __builtin_bit_cast(vl_f16x2, r);   // both halves are written
         v = __builtin_amdgcn_cvt_sr_f16_f32(v, a, r, false);
         v = __builtin_amdgcn_cvt_sr_f16_f32(v, b, r << 13, true);
         return __builtin_bit_cast(unsigned, v);
@@ -234,6 +259,73 @@ __device__ __forceinline__ void rs_wait_vm() {
 // the loop, where the count also covers the next tile's prefetch.
 __device__ __forceinline__ void rs_wait_all() { __builtin_amdgcn_s_waitcnt(0); }
 
+// ---- moment partials of the stored values (stage epilogue and k_moment_rs,
+// which must give the same bits for the same state) -------------------------
+// Weights in a small LDS table (read where they are used, so that no
+// registers hold them across the passes): [0, 8) w_v of nodes 0..7 (a lane
+// takes the four of its unit's quad q = ln & 1, the same in every pass),
+// [8, 16) the pairs (w_u[j], w_u[j + 4]) of the reduction lane 4 ix + j.
+// Written by the first 16 threads of the workgroup, then one barrier.
+constexpr int kRsWeights = 16;
+__device__ __forceinline__ void rs_weights_fill(float *wt, ConstOpTables &tab, int tid) {
+    if (tid < 8) wt[tid] = tab.wv[tid];
+    else if (tid < 16) wt[tid] = tab.wu[(tid - 8) / 2 + 4 * ((tid - 8) & 1)];
+}
+
+// Moment of unit u0 (pass p, stored words w): w_v-weighted values; a
+// half-row unit keeps their sum at rw[u0], a tail unit its four rows' values
+// at rw[TWO + 4 m ..] (units past the last write to a dummy slot).
+template <int N, int S>
+__device__ __forceinline__ void rs_unit_moment(float *rw, int p, int u0, vl_u32x2 w, const float *wtab,
+                                               ConstOpTables &tab) {
+    using SH = RsShape<N>;
+    using PK = RsPack<S>;
+    const bool tail_pass = N == 9 && 64 * p + 63 >= SH::UH;
+    vl_f32x4 wt = *reinterpret_cast<const vl_f32x4 *>(wtab + 4 * (u0 & 1));
+    if (tail_pass) {
+        const float wv8 = tab.wv[N - 1];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) wt[t] = u0 >= SH::UH ? wv8 : wt[t];
+    }
+    vl_f32x4 mv;
+    mv[0] = wt[0] * PK::lo(w.x);
+    mv[1] = wt[1] * PK::hi(w.x);
+    mv[2] = wt[2] * PK::lo(w.y);
+    mv[3] = wt[3] * PK::hi(w.y);
+    if (!tail_pass || u0 < SH::UH) rw[u0] = (mv[0] + mv[1]) + (mv[2] + mv[3]);
+    if (tail_pass && u0 >= SH::UH)
+        *reinterpret_cast<vl_f32x4 *>(rw + (u0 < SH::NUNIT ? SH::TWO + 4 * (u0 - SH::UH) : SH::RDUMMY)) = mv;
+}
+
+// Moment of x-node ix = ln >> 2 of the element from the unit moments (after
+// a wave sync): sum over rows (ix, iu) of w_u (half-row sums + tail value),
+// complete on lanes 4 ix (lanes with ix >= n return garbage).
+template <int N>
+__device__ __forceinline__ float rs_element_moment(const float *rw, int ln, const float *wtab, ConstOpTables &tab) {
+    using SH = RsShape<N>;
+    const int rix = min(ln >> 2, N - 1), rj = ln & 3;
+    const int ra = rix * N + rj, rb = ra + 4;
+    const vl_f32x2 ha = *reinterpret_cast<const vl_f32x2 *>(rw + 2 * ra);
+    const vl_f32x2 hb = *reinterpret_cast<const vl_f32x2 *>(rw + 2 * rb);
+    float ta = 0.f, tb = 0.f;
+    if (N == 9) {
+        ta = rw[SH::TWO + ra];
+        tb = rw[SH::TWO + rb];
+    }
+    const vl_f32x2 wu = *reinterpret_cast<const vl_f32x2 *>(wtab + 8 + 2 * rj);
+    float a = wu[0] * ((ha[0] + ha[1]) + ta) + wu[1] * ((hb[0] + hb[1]) + tb);
+    if (N == 9) {   // row (ix, 8) on lane j = 0
+        const int rc = rix * N + 8;
+        const vl_f32x2 hc = *reinterpret_cast<const vl_f32x2 *>(rw + 2 * rc);
+        const float c8 = tab.wu[N - 1] * ((hc[0] + hc[1]) + rw[SH::TWO + rc]);
+        a += rj == 0 ? c8 : 0.f;
+    }
+    // sum over the four lanes of the x-node (quad_perm 1,0,3,2 then 2,3,0,1)
+    a += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, a), 0xB1, 0xf, 0xf, false));
+    a += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, a), 0x4E, 0xf, 0xf, false));
+    return a;
+}
+
 template <int N>
 struct RsLoads {
     vl_u32x4 row[RsShape<N>::NRI];   // own rows: nodes 0..7 of line (ix, iu)
@@ -261,14 +353,20 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     using F = MxFmt<S>;
     using PK = RsPack<S>;
     constexpr int TV = SH::TV, EP = SH::EP, ROWS = SH::ROWS, NRI = SH::NRI, RSD = SH::RSD, PSD = SH::PSD,
-                  ESD = SH::ESD, XR = SH::XR, XE = SH::XE, GMAX = SH::GMAX, NNB = SH::NNB;
+                  ESD = SH::ESD, GMAX = SH::GMAX, NNB = SH::NNB, UH = SH::UH, NUNIT = SH::NUNIT,
+                  NPASS = SH::NPASS, XR = SH::XR, XE = SH::XE, TAIL = SH::TAIL, TF = SH::TF, RW = SH::RW, TWO = SH::TWO,
+                  RDUMMY = SH::RDUMMY;
 
-    __shared__ __attribute__((aligned(16))) unsigned int brick[TV * ESD];
+    // bricks of the tile's elements, then (n = 9) node 8 of every row of the
+    // stage input, rows in order, zero past the last row: the tail units'
+    // stage-input values
+    constexpr int TFD = TF / 2;
+    __shared__ __attribute__((aligned(16))) unsigned int brick[TV * ESD + TV * TFD];
+    // L of this wave's element, FP32 rows (ix, iu) of v-slots 0..11: the x pass
+    // writes A_x g, the u/v pass replaces it by L (node 8 at slot 8)
     __shared__ __attribute__((aligned(16))) float xa[TV * XE];
-    __shared__ float red[TV * ROWS];   // per-row moment sums, wave-private rows
-    // per-lane sink of the 16-byte stores of lanes that hold no output node
-    // (shared by the waves: never read)
-    __shared__ __attribute__((aligned(16))) float sink[64 * 4];
+    // per-wave scratch: sink of the passes' discarded lanes, then moment sums
+    __shared__ __attribute__((aligned(16))) float rwk[TV * RW];
     // f^n of this wave's element, staged by direct-to-LDS loads: rows (16 B
     // each) then the tail dwords
     constexpr int FNR = ROWS * 4, FNW = ((ROWS * 4 + (N == 9 ? 64 : 0)) + 3) & ~3;
@@ -289,6 +387,8 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const int64_t sxe = s.sxe;
     unsigned int *bk = brick + el * ESD;
     float *xw = xa + el * XE;
+    uint16_t *tfw = reinterpret_cast<uint16_t *>(brick + TV * ESD + el * TFD);
+    float *rw = rwk + el * RW;
 
     auto ebase = [&](int ex_, int eu_, int ev_) { return (int64_t)ex_ * sxe + ((int64_t)eu_ * vrow + ev_) * EP; };
     auto xmap = [&](int e) {
@@ -348,9 +448,14 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
         if (lg == 2) vsel = av >= 0.0 ? kSelPos : kSelNeg;
 #pragma unroll
         for (int su = 0; su < N; ++su) xposm |= (unodes[eu * N + su] >= 0.0 ? 1u : 0u) << su;
-        // never-written brick slots stay zero (finite) for the whole kernel
+        // never-written brick slots stay zero (finite) for the whole kernel,
+        // and so do the tail inputs past the last row
         for (int i = ln; i < ESD; i += 64) bk[i] = 0u;
+        if (N == 9 && ln < TF - ROWS) tfw[ROWS + ln] = 0;
     }
+    __shared__ __attribute__((aligned(16))) float wtab[kRsWeights];
+    rs_weights_fill(wtab, *(ConstOpTables *)tabp, tid);
+    __syncthreads();
 
     // ---- global loads of one tile ------------------------------------------
     // Raw buffer loads over whole state arrays (fewer than 2^31 bytes, checked
@@ -437,6 +542,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                                            : halves(R.w, 1, R.x, 0);             // slot 9 = node 0
                 const unsigned w5 = halves(d.vnl[k], N == 9 ? (r & 1) : 1, d.vnr[k], 0);   // slots 10, 11: v faces
                 put(ix, iu, R, w4, w5, true);
+                if (N == 9) tfw[r] = (uint16_t)w4;               // node 8 for the tail units
                 if (iu == 0) put(ix, 9, R, w4, w5, true);       // u-slot 9: copy of node 0
                 if (ix == 0) put(9, iu, R, w4, w5, true);       // x-slot 9: copy of node 0
                 if (N == 8 && iu == N - 1) put(ix, 8, R, w4, w5, true);   // slot 8: copy of node n-1
@@ -492,7 +598,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             // node store to their sink.
             const int rduv = min(lj, 11) * RSD + 2 * min(lg, 2);   // u-slot lj, v-slots 4 lg ..
             const bool wr = lj < N && lg <= GMAX;
-            float *const snk = sink + ln * 4;
+            float *const snk = rw + ln * 4;
             const unsigned selp = lg == 2 ? kSelPos : kSelKeep, seln = lg == 2 ? kSelNeg : kSelKeep;
             // ---- x: X[ix][iu][v-slot] = A_x g (the x speed is applied in the u/v pass)
 #pragma unroll
@@ -538,7 +644,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                 task_fence();
             }
             wave_sync_lds();
-            // ---- epilogue: whole rows ----------------------------------------
+            // ---- epilogue: one unit of four stored values per lane and pass --
             const int64_t bo = ebase(ex, eu, ev);
             // stochastic rounding: this lane's word stream for this element
             unsigned rstate = sround ? (hash32(((unsigned)((ex * (s.nu + 2) + eu) * vrow + ev) << 6 | (unsigned)ln) ^
@@ -553,99 +659,95 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
                 else
                     rs_wait_vm<0>();
             }
-            // Branch-free passes: lanes past the last row recompute it and their
-            // stores go to an out-of-range offset (dropped), so every pass issues
-            // the same stores (the next tile's wait for its prefetch counts them).
-#pragma unroll 1
-            for (int k = 0; k < NRI; ++k) {
-                const int r0 = ln + 64 * k;
-                const bool rok = r0 < ROWS;
-                const int r = rok ? r0 : ROWS - 1;
-                {
-                    vl_u32x4 fnr = {0u, 0u, 0u, 0u};
-                    unsigned fnt = 0u;
-                    if (MODE == kModeStage) {
-                        const unsigned *st0 = fnst + el * FNW;
-                        fnr = *reinterpret_cast<const vl_u32x4 *>(st0 + r * 4);
-                        fnt = N == 9 ? st0[FNR + (r >> 1)] : 0u;
-                    }
-                    const int iu = r - (r / N) * N;
-                    const int ix = r / N;
-                    const float *xp = xw + r * XR;
-                    const unsigned *br = bk + ix * PSD + iu * RSD;    // stage input row, slots 0..8
-                    const unsigned nw[4] = {fnr.x, fnr.y, fnr.z, fnr.w};
-                    ConstOpTables &tab = *(ConstOpTables *)tabp;
-                    // pairs of values: combine, round, moment of the stored pair
-                    auto pair = [&](float l0, float l1, unsigned fiw, unsigned fnw, float &m, int j) {
-                        float o0, o1;
-                        if (MODE == kModeStage1) {
-                            o0 = PK::lo(fiw) + c2 * l0;
-                            o1 = PK::hi(fiw) + c2 * l1;
-                        } else {
-                            o0 = (c0 * PK::lo(fnw) + c1 * PK::lo(fiw)) + c2 * l0;
-                            o1 = (c0 * PK::hi(fnw) + c1 * PK::hi(fiw)) + c2 * l1;
-                        }
-                        unsigned wv2;
-                        if (sround) {
-                            rstate = xorshift32(rstate);
-                            wv2 = PK::sr2(o0, o1, rstate);
-                        } else {
-                            wv2 = F::enc2(o0, o1);
-                        }
-                        m += tab.wv[j] * PK::lo(wv2);
-                        m += tab.wv[j + 1] * PK::hi(wv2);
-                        return wv2;
-                    };
-                    float m = 0.f;
-                    vl_u32x4 w;
-                    {
-                        const vl_f32x4 l0 = *reinterpret_cast<const vl_f32x4 *>(xp);
-                        const vl_u32x2 f01 = *reinterpret_cast<const vl_u32x2 *>(br);
-                        w.x = pair(l0[0], l0[1], f01.x, nw[0], m, 0);
-                        w.y = pair(l0[2], l0[3], f01.y, nw[1], m, 2);
-                    }
-                    {
-                        const vl_f32x4 l1 = *reinterpret_cast<const vl_f32x4 *>(xp + 4);
-                        const vl_u32x2 f23 = *reinterpret_cast<const vl_u32x2 *>(br + 2);
-                        w.z = pair(l1[0], l1[1], f23.x, nw[2], m, 4);
-                        w.w = pair(l1[2], l1[3], f23.y, nw[3], m, 6);
-                    }
-                    buf_st16_nt(rout, rok ? (int)(bo * 2) + r * 16 : kOob, w);
-                    if (N == 9) {
-                        const float fi8 = PK::lo(br[4]);
-                        const float fn8 = (r & 1) ? PK::hi(fnt) : PK::lo(fnt);
-                        const float o8 = MODE == kModeStage1 ? fi8 + c2 * xp[8] : (c0 * fn8 + c1 * fi8) + c2 * xp[8];
-                        unsigned t8;
-                        if (sround) {
-                            rstate = xorshift32(rstate);
-                            t8 = PK::sr1(o8, rstate);
-                        } else {
-                            t8 = F::enc(o8);
-                        }
-                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)t8, rout,
-                                                              rok ? (int)(bo * 2) + (8 * ROWS + r) * 2 : kOob,
-                                                              0, 0);
-                        m += tab.wv[8] * PK::lo(t8);
-                    }
-                    // v-moment of the stored row (its u weight is applied in the sum below)
-                    if (rok) red[el * ROWS + r] = m;
+            const unsigned *fnw = fnst + el * FNW;
+            const int obase = (int)(bo * 2);
+            // the lane index as an opaque value: the per-lane addresses of the
+            // epilogue are formed here, not hoisted out of the tile loop
+            // (where they would hold registers across the passes)
+            int lno = ln;
+            asm volatile("" : "+v"(lno));
+            lno &= 63;
+#pragma unroll
+            for (int p = 0; p < NPASS; ++p) {
+                const int u0 = 64 * p + lno;
+                // units past the last one recompute it; their stores are dropped
+                const int u = min(u0, NUNIT - 1);
+                // L: a half row is one 16-byte read at row r, slot 4q; a tail unit
+                // takes slot 8 of rows 4m .. 4m+3 (rows past the last read a
+                // clamped place; their values are not stored)
+                vl_f32x4 l4;
+                if (N == 9 && 64 * p + 63 >= UH) {
+                    const bool tl = u >= UH;
+                    const int ua = tl ? 4 * (u - UH) * XR + 8 : (u >> 1) * XR + 4 * (u & 1);
+                    l4 = *reinterpret_cast<const vl_f32x4 *>(xw + ua);
+                    const float *pb = xw + min(ua, XE - 1 - 3 * XR);
+                    const float b1 = pb[XR], b2 = pb[2 * XR], b3 = pb[3 * XR];
+                    l4[1] = tl ? b1 : l4[1];
+                    l4[2] = tl ? b2 : l4[2];
+                    l4[3] = tl ? b3 : l4[3];
+                } else {
+                    l4 = *reinterpret_cast<const vl_f32x4 *>(xw + (u >> 1) * XR + 4 * (u & 1));
                 }
+                // stage-input values: the brick row (ix, iu), slots 4q .. 4q+3,
+                // of a half row; node 8 of rows 4m .. 4m+3 for a tail unit
+                const int r = u >> 1, ix = r / N;
+                const int fa = u < UH ? el * ESD + ix * (PSD - N * RSD) + r * RSD + 2 * (u & 1)
+                                      : TV * ESD + el * TFD + 2 * (u - UH);
+                const vl_u32x2 fi = *reinterpret_cast<const vl_u32x2 *>(brick + fa);
+                vl_u32x2 fv = {0u, 0u};
+                if (MODE == kModeStage)
+                    fv = *reinterpret_cast<const vl_u32x2 *>(fnw + 2 * u);
+                float o[4];
+                {
+                    const unsigned fiw[2] = {fi.x, fi.y}, fnv[2] = {fv.x, fv.y};
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (MODE == kModeStage1) {
+                            o[2 * h] = PK::lo(fiw[h]) + c2 * l4[2 * h];
+                            o[2 * h + 1] = PK::hi(fiw[h]) + c2 * l4[2 * h + 1];
+                        } else {
+                            o[2 * h] = (c0 * PK::lo(fnv[h]) + c1 * PK::lo(fiw[h])) + c2 * l4[2 * h];
+                            o[2 * h + 1] = (c0 * PK::hi(fnv[h]) + c1 * PK::hi(fiw[h])) + c2 * l4[2 * h + 1];
+                        }
+                    }
+                }
+                vl_u32x2 w;
+                if (sround) {
+                    rstate = xorshift32(rstate);
+                    w.x = PK::sr2(o[0], o[1], rstate);
+                    rstate = xorshift32(rstate);
+                    w.y = PK::sr2(o[2], o[3], rstate);
+                } else {
+                    w.x = F::enc2(o[0], o[1]);
+                    w.y = F::enc2(o[2], o[3]);
+                }
+                // stores: whole units, and the one unit that holds the last
+                // row's node 8 and padding past it (n = 9) as a single value
+                constexpr bool has_partial = NUNIT * 4 > (N == 9 ? TAIL + ROWS : TAIL);
+                const bool whole = u0 < (has_partial ? NUNIT - 1 : NUNIT);
+                __builtin_amdgcn_raw_buffer_store_b64(w, rout, whole ? obase + 8 * u : kOob, 0, kCachePolicyNT);
+                if (has_partial && 64 * p + 63 >= NUNIT - 1)
+                    __builtin_amdgcn_raw_buffer_store_b16((uint16_t)w.x, rout,
+                                                          u0 == NUNIT - 1 ? obase + 8 * u : kOob, 0, 0);
+                if (partials) rs_unit_moment<N, S>(rw, p, u0, w, wtab, *(ConstOpTables *)tabp);
+                // one pass at a time (hoisting the next pass's reads over this one
+                // would hold more registers than the kernel has)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         // moment partial of this wave's element (one slot per tile element,
         // zero for the unused elements of a partial chunk); no barrier
         if (partials) {
             wave_sync_lds();
-            if (ln < N) {
-                double a = 0.0;
-                if (active) {
-                    ConstOpTables &tab = *(ConstOpTables *)tabp;
-#pragma unroll
-                    for (int iu = 0; iu < N; ++iu) a += (double)(tab.wu[iu] * red[el * ROWS + ln * N + iu]);
-                }
+            int lno = ln;
+            asm volatile("" : "+v"(lno));
+            lno &= 63;
+            float a = rs_element_moment<N>(rw, lno, wtab, *(ConstOpTables *)tabp);
+            const int rix = lno >> 2;
+            if ((lno & 3) == 0 && rix < N) {
                 const int64_t slot =
-                    (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + ln;
-                s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
+                    (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + rix;
+                s.partials[slot] = make_double2(active ? (double)a * s.inv_scale : 0.0, 0.0);
             }
         }
         if (has_next) fill(nxt);
@@ -653,15 +755,16 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
 }
 
 // Moment partials of a row-split state (first stage of a loop), in the
-// partial layout of k_stage_rs (one slot per tile element): one workgroup per
-// tile (ex, eu, 4 v-elements), one wave per element.
+// partial layout of k_stage_rs (one slot per tile element) and with its
+// arithmetic (rs_unit_moment, rs_element_moment), so that both give the same
+// partials for the same stored values: one workgroup per tile (ex, eu,
+// 4 v-elements), one wave per element.
 template <int N, int S>
 __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict__ f, StageArgs s,
                                                       const OpTables *__restrict__ tabp) {
     using SH = RsShape<N>;
-    using PK = RsPack<S>;
-    constexpr int TV = SH::TV, EP = SH::EP, ROWS = SH::ROWS, NRI = SH::NRI;
-    __shared__ float red[TV * ROWS];   // wave-private rows
+    constexpr int TV = SH::TV, EP = SH::EP, RW = SH::RW, NPASS = SH::NPASS, NUNIT = SH::NUNIT;
+    __shared__ __attribute__((aligned(16))) float rwk[TV * RW];   // wave-private moment scratch
     const int tid = threadIdx.x, ln = tid & 63, el = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile % s.nchunk;
@@ -670,34 +773,28 @@ __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict
     const int ex = 1 + t2 / s.nu;
     const int v0 = 1 + chunk * TV;
     const int nvalid = min(TV, s.nv + 1 - v0);
-    ConstOpTables &tab = *(ConstOpTables *)tabp;
-    if (el < nvalid) {
+    __shared__ __attribute__((aligned(16))) float wtab[kRsWeights];
+    rs_weights_fill(wtab, *(ConstOpTables *)tabp, tid);
+    __syncthreads();
+    float *rw = rwk + el * RW;
+    const bool active = el < nvalid;
+    if (active) {
         const int64_t b = (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + v0 + el) * EP;
 #pragma unroll
-        for (int k = 0; k < NRI; ++k) {
-            const int r = ln + 64 * k;
-            if (r < ROWS) {
-                const vl_u32x4 R = *reinterpret_cast<const vl_u32x4 *>(f + b + r * 8);
-                const unsigned ww[4] = {R.x, R.y, R.z, R.w};
-                float m = 0.f;
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    m += tab.wv[2 * h] * PK::lo(ww[h]);
-                    m += tab.wv[2 * h + 1] * PK::hi(ww[h]);
-                }
-                if (N == 9) m += tab.wv[8] * PK::lo((unsigned)f[b + 8 * ROWS + r]);
-                red[el * ROWS + r] = tab.wu[r - (r / N) * N] * m;
-            }
+        for (int p = 0; p < NPASS; ++p) {
+            const int u0 = 64 * p + ln;
+            const int u = min(u0, NUNIT - 1);
+            const vl_u32x2 w = *reinterpret_cast<const vl_u32x2 *>(f + b + 4 * u);
+            rs_unit_moment<N, S>(rw, p, u0, w, wtab, *(ConstOpTables *)tabp);
         }
     }
     wave_sync_lds();
-    if (ln < N) {
-        double a = 0.0;
-        if (el < nvalid)
-            for (int iu = 0; iu < N; ++iu) a += (double)red[el * ROWS + ln * N + iu];
+    const float a = rs_element_moment<N>(rw, ln, wtab, *(ConstOpTables *)tabp);
+    const int rix = ln >> 2;
+    if ((ln & 3) == 0 && rix < N) {
         const int tiles_per_ex = s.nu * s.nchunk;
-        const int64_t slot = (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * TV + el) * N + ln;
-        s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
+        const int64_t slot = (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * TV + el) * N + rix;
+        s.partials[slot] = make_double2(active ? (double)a * s.inv_scale : 0.0, 0.0);
     }
 }
 
