@@ -177,15 +177,17 @@ struct StageLds {
 };
 
 // Workgroups per CU the stage kernel is compiled for.  The element-major FP64
-// kernel at n = 9 fits 4 per CU (40.5 KB of LDS, 128 registers); the others
-// are compiled for 3 (the reference-layout kernels would spill at 128).
-template <int N, int S, int L>
+// stage-2/3 kernel at n = 9 fits 4 per CU (40 KB of LDS, 128 registers).  The
+// others are compiled for 3: the reference-layout kernels would spill at 128,
+// and the FP64 first stage (no f^n to hide) runs 5 % faster with the larger
+// register budget at 3 per CU than at 4 (measured on MI355X).
+template <int N, int S, int L, int MODE = kModeStage>
 constexpr int stage_min_blocks() {
-    return (N == 9 && S == 0 && L == kElemLayout) ? 4 : 3;
+    return (N == 9 && S == 0 && L == kElemLayout && MODE == kModeStage) ? 4 : 3;
 }
 
 template <int N, int MODE, int S, int L, bool ADT = false>
-__global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S, L>())) void k_stage(
+__global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S, L, MODE>())) void k_stage(
     const typename Store<S>::E *__restrict__ fin,  // stage input (interior; ghosts too when !periodic_x)
     const typename Store<S>::E *__restrict__ fgh,  // array whose u/v ghost planes are used
     const typename Store<S>::E *__restrict__ fn,   // f^n for MODE == kModeStage (may alias fout)
