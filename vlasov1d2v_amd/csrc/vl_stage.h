@@ -16,19 +16,20 @@
 // n^3 nodes of each element, TV = floor(256 / n^2), so every direction has
 // n*P <= 256 lines of n nodes (P = TV*n v-nodes).  One workgroup of 256
 // threads owns a pencil of tiles (eu, v-chunk, x-segment) and walks it along
-// x.  While it computes tile ex from LDS, the loads of tile ex+1, of its u-face
-// halo and of the first x-plane of tile ex+2 (the right x-halo one step ahead)
-// are already in flight in registers; the left x-halo is the last x-plane of
-// the previous tile, copied inside LDS.  So every element of the stage input
-// is fetched from HBM once; u-halos are read from L2 because the neighbouring
-// pencils (eu +- 1) run on the same XCD at the same time.
+// x.  While it computes tile ex from LDS, the loads of tile ex+1 and of its
+// u-face halo are in flight in registers.  The right x-halo of tile ex (the
+// first x-plane of element ex+1) is loaded first at the start of the tile and
+// used by the last line phase; the next tile's rows, loaded right after it,
+// find that plane in L2.  The left x-halo is the last x-plane of the previous
+// tile (a register per thread).  u-halos come from the neighbouring pencils
+// (eu +- 1), which run on the same XCD at about the same time.
 //
-// Per tile each thread contracts one x-line, one u-line and one v-line with
+// Per tile each thread contracts one u-line, one v-line and one x-line with
 // the n x n volume operator (kernel-argument memory, read through the scalar
-// cache) and accumulates J_d * L_d into an LDS output tile in the reference's
-// order ((Lx Jx + Lu Ju) + Lv Jv).  The epilogue combines with the stage
-// inputs, writes rows that are contiguous along v, and reduces the moment
-// partial of the tile.
+// cache) and accumulates J_d * L_d into an LDS output tile as
+// ((Lu Ju + Lv Jv) + Lx Jx).  The epilogue combines with the stage inputs,
+// writes rows that are contiguous along v, and reduces the moment partial of
+// the tile.
 //
 // Background of the reduced forms.  f0(u, v) is x-independent, so
 // L(f) = L(g)/scale + L(f0) with L_x(f0) = 0, L_u(f0) = a_u * G+-(u, v)
