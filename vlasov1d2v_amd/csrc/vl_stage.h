@@ -177,6 +177,38 @@ struct StageLds {
     static_assert(BG || (o_uh % 8 == 0), "the FP64 moment scratch aliases uh");
 };
 
+// Sum of the P column partials red[ix * P + q] of each x node ix of a tile:
+// groups of 4 columns, then the groups in order.  The vector-ALU stage kernel
+// and k_moment_partials both reduce with it, so that the partials recomputed
+// from a stored state equal those the stage kernel wrote, bit for bit (a
+// step gives the same result whether it reuses the previous step's partials
+// or recomputes them).  Block-wide: call from every thread once red[] is
+// written; the sum is returned to threads tid < N.
+template <int N, int P>
+__device__ __forceinline__ double tile_column_sum(double *red, int tid) {
+    constexpr int G = 4;
+    constexpr int NG = (P + G - 1) / G;
+    __syncthreads();
+    const bool g_on = tid < N * NG;
+    const int gix = g_on ? tid / NG : 0;
+    const int gg = g_on ? tid - gix * NG : 0;
+    double gacc = 0.0;
+    if (g_on) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int q = gg * G + k;
+            if (q < P) gacc += red[gix * P + q];
+        }
+    }
+    __syncthreads();
+    if (g_on) red[gix * P + gg] = gacc;
+    __syncthreads();
+    double acc = 0.0;
+    if (tid < N)
+        for (int g = 0; g < NG; ++g) acc += red[tid * P + g];
+    return acc;
+}
+
 // Workgroups per CU the stage kernel is compiled for.  The element-major FP64
 // stage-2/3 kernel at n = 9 fits 4 per CU (40 KB of LDS, 128 registers).  The
 // others are compiled for 3: the reference-layout kernels would spill at 128,
@@ -247,10 +279,8 @@ __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store
         }
         red[tid] = acc;
     }
-    __syncthreads();
+    const double acc = tile_column_sum<N, P>(red, tid);
     if (tid < N) {
-        double acc = 0.0;
-        for (int q = 0; q < P; ++q) acc += red[tid * P + q];
         const int tiles_per_ex = s.nu * s.nchunk;
         const int64_t slot = ((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * N + tid;
         s.partials[slot] = make_double2(ST::kDelta ? acc * s.inv_scale : acc, 0.0);
