@@ -16,13 +16,14 @@
 // n^3 nodes of each element, TV = floor(256 / n^2), so every direction has
 // n*P <= 256 lines of n nodes (P = TV*n v-nodes).  One workgroup of 256
 // threads owns a pencil of tiles (eu, v-chunk, x-segment) and walks it along
-// x.  While it computes tile ex from LDS, the loads of tile ex+1 and of its
-// u-face halo are in flight in registers.  The right x-halo of tile ex (the
-// first x-plane of element ex+1) is loaded first at the start of the tile and
-// used by the last line phase; the next tile's rows, loaded right after it,
-// find that plane in L2.  The left x-halo is the last x-plane of the previous
-// tile (a register per thread).  u-halos come from the neighbouring pencils
-// (eu +- 1), which run on the same XCD at about the same time.
+// x, in the direction of the pencil's x-line speeds (the u nodes of element
+// eu): the upwind x halo of each tile is then the end x-plane of the previous
+// tile of the walk, a register per thread, and no x halo is loaded.  (A pencil
+// whose element holds both signs walks towards +x and loads the right x-plane
+// at the start of each tile.)  While it computes a tile from LDS, the loads
+// of the next tile of the walk and of its u-face halo are in flight in
+// registers.  u-halos come from the neighbouring pencils (eu +- 1), which run
+// on the same XCD at about the same time.
 //
 // Per tile each thread contracts one u-line, one v-line and one x-line with
 // the n x n volume operator (kernel-argument memory, read through the scalar
