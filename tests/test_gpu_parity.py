@@ -250,6 +250,38 @@ def test_row_split_layout_conversion(cuda, n):
     assert np.array_equal(b[elem * pitch + inner], a.reshape(-1))
 
 
+@pytest.mark.parametrize("storage,n", [("f64", 9), ("f64", 4), ("f32", 7), ("bf16", 9), ("f16", 5)])
+def test_element_major_layout_conversion(cuda, storage, n):
+    """Reference -> element-major -> reference is exact, and the element-major
+    array holds node (ix, iu, iv) of element (ex, eu, ev) at element * pitch +
+    (iu n + iv) n + ix for FP64 / FP32 states (x fastest) and
+    (ix n + iu) n + iv for BF16 / FP16 states."""
+    import torch
+    from vlasov1d2v_amd import Run, _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    nx, nu, nv = 2, 3, 2
+    r = Run(n, nx, nu, nv, storage=storage)
+    g = r.grids.geometry(STORAGE_CODES[storage])
+    shape = r.grids.shape
+    # distinct bit patterns for every node, compared as integers
+    itype = {"f64": torch.int64, "f32": torch.int32, "bf16": torch.int16, "f16": torch.int16}[storage]
+    vals = (torch.arange(int(np.prod(shape)), dtype=torch.int64) % 30000 + 1).to(itype)
+    src = vals.view(r.dist.arr.dtype).reshape(shape).to(cuda)
+    em = torch.zeros(g.state_elems_layout(_lib.VL_LAYOUT_ELEMENT_MAJOR), dtype=src.dtype, device=cuda)
+    g.convert_layout(src, _lib.VL_LAYOUT_REFERENCE, em, _lib.VL_LAYOUT_ELEMENT_MAJOR)
+    back = torch.zeros_like(src)
+    g.convert_layout(em, _lib.VL_LAYOUT_ELEMENT_MAJOR, back, _lib.VL_LAYOUT_REFERENCE)
+    torch.cuda.synchronize()
+    assert torch.equal(back.view(itype).cpu(), src.view(itype).cpu())
+    pitch = (n ** 3 + 7) // 8 * 8
+    a = src.view(itype).cpu().numpy()
+    b = em.view(itype).cpu().numpy()
+    ex, ix, eu, iu, ev, iv = np.indices(shape).reshape(6, -1)
+    elem = (ex * (nu + 2) + eu) * (nv + 2) + ev
+    inner = (iu * n + iv) * n + ix if storage in ("f64", "f32") else (ix * n + iu) * n + iv
+    assert np.array_equal(b[elem * pitch + inner], a.reshape(-1))
+
+
 def test_resident_layout_choice(cuda):
     """The library reports the row-split layout for the 16-bit forms with
     n = 8, 9 and the element-major layout otherwise."""

@@ -53,11 +53,22 @@ __device__ __forceinline__ void set_lane(vl_u32x4 &v, int k, unsigned int b) {
 
 // Layouts of a state array.  kRefLayout is the reference's C-order
 // (NX+2, n, NU+2, n, NV+2, n).  kElemLayout ("element-major") keeps the n^3
-// nodes of each element contiguous, (ix, iu, iv) inside, elements ordered
-// (ex, eu, ev) and padded to a pitch of EP = round_up(n^3, 8) values so that
-// every element starts 16-byte aligned for all storage types.
+// nodes of each element contiguous, elements ordered (ex, eu, ev) and padded
+// to a pitch of EP = round_up(n^3, 8) values so that every element starts
+// 16-byte aligned for all storage types.  Node order inside an element:
+// FP64 / FP32 states put x fastest, node (ix, iu, iv) at (iu n + iv) n + ix,
+// so that an element's u-face (iu fixed) is one run of n^2 values and its
+// v-face nine runs of n: those are the faces a stage tile reads from the
+// neighbouring pencils (the x halo comes from the tile's own walk).  BF16 /
+// FP16 states keep (ix n + iu) n + iv, the order their matrix-core kernel
+// builds its operand blocks from.
 enum { kRefLayout = 0, kElemLayout = 1 };
 __host__ __device__ constexpr int elem_pitch(int n) { return ((n * n * n + 7) / 8) * 8; }
+// storage codes as in Store<S> below: 0 FP64, 1 FP32, 2 BF16, 3 FP16
+__host__ __device__ constexpr bool elem_x_minor(int storage) { return storage == 0 || storage == 1; }
+__host__ __device__ constexpr int elem_node(bool x_minor, int n, int ix, int iu, int iv) {
+    return x_minor ? (iu * n + iv) * n + ix : (ix * n + iu) * n + iv;
+}
 __device__ __forceinline__ vl_u32x4 buf_ld16(__amdgpu_buffer_rsrc_t r, int off) {
     return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
 }

@@ -146,15 +146,18 @@ __device__ __forceinline__ Pencil decode_pencil(const StageArgs &s, int p) {
 // separate __shared__ arrays; the ensemble kernel carves them out of one
 // buffer (byte offsets below, the 16-byte aligned arrays first).
 //   ft, acc  tile values and RHS accumulator, element-major like the state's
-//            element-major layout: node (el, ix, iu, iv) at el EP + (ix N + iu) N + iv,
-//            so that the 16-byte rows of that layout move as whole vectors
-//   vh       v-halo nodes of the tile rows r = ix N + iu: [side][r]
-//   uh       u-halo faces [side][el][ix][iv]
+//            element-major layout: node (el, ix, iu, iv) at
+//            el EP + elem_node(elem_x_minor(S), N, ix, iu, iv), so that the
+//            16-byte runs of that layout move as whole vectors
+//   vh       v-halo nodes of the tile's v-line rows: [side][a N + b], the row
+//            (ix, iu) = (a, b), x-minor order (iu, ix) = (a, b)
+//   uh       u-halo faces [side][el][ix][iv], x-minor order [side][el][iv][ix]
 // (the x halos stay in registers: one value per x-line and thread)
-// With one line per thread in the order (el, a, b) the x-line reads of ft and
-// the v-line reads (9 consecutive values, lanes 9 apart) are free of LDS bank
-// conflicts on 64 x 4-byte banks, and the epilogue's 16-byte accesses are
-// contiguous across lanes.
+// With one line per thread in the order (el, a, b), LDS reads of the x-lines
+// and v-lines (old order) or of the x-lines and u-lines (x-minor order) are
+// free of bank conflicts on 64 x 4-byte banks; the third direction has a
+// two-way conflict.  The epilogue's 16-byte accesses are contiguous across
+// lanes.
 template <int N, int S>
 struct StageLds {
     using C = typename Store<S>::C;
@@ -163,8 +166,8 @@ struct StageLds {
     static constexpr int EP = elem_pitch(N);
     static constexpr int n_ft = TileShape<N>::TV * EP;   // tile values, element-major
     static constexpr int n_acc = n_ft;                   // RHS accumulator, same layout
-    static constexpr int n_vh = 2 * N * N;               // v halo [side][ix*N + iu]
-    static constexpr int n_uh = 2 * N * P;               // u halo [side][el][ix][iv]
+    static constexpr int n_vh = 2 * N * N;               // v halo [side][row]
+    static constexpr int n_uh = 2 * N * P;               // u halo [side][el][line]
     static constexpr int n_bgs = BG ? 3 * N * P : 1;     // background slab [table][iu][q]
     static constexpr int n_red = BG ? N * P : 1;         // moment scratch (the FP64 form reuses uh)
     static constexpr size_t o_ft = 0;
@@ -273,7 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store
 #pragma unroll
             for (int iu = 0; iu < N; ++iu) {
                 const int64_t off = L == kElemLayout
-                    ? (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + v0 + el) * EP + (ix * N + iu) * N + iv
+                    ? (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + v0 + el) * EP +
+                          elem_node(elem_x_minor(S), N, ix, iu, iv)
                     : (int64_t)ex * s.sxe + ix * s.sx + (int64_t)eu * s.sue + iu * s.su + gv0 + q;
                 acc += (double)ST::ld(f, off) * tab.wq[iu * N + iv];
             }
@@ -290,8 +294,9 @@ __global__ __launch_bounds__(kBlock) void k_moment_partials(const typename Store
 
 // Layout conversion of a state array (element width W bytes via type U):
 // to_elem = 1: reference -> `lay`, 0: `lay` -> reference, where `lay` is
-// kElemLayout (element-major) or 2 (row-split, n = 8 or 9).  Padding values
-// of the element-wise arrays are not written.
+// kElemLayout (element-major; node order by width: x fastest for the 8- and
+// 4-byte FP64 / FP32 states, see elem_x_minor) or 2 (row-split, n = 8 or 9).
+// Padding values of the element-wise arrays are not written.
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__ src, U *__restrict__ dst,
                                                            int to_elem, int lay, int n, int nu, int nv,
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__
         t /= (nu + 2);
         const int ix = (int)(t % n);
         const int64_t ex = t / n;
-        const int inner = lay == kElemLayout ? (ix * n + iu) * n + iv
+        const int inner = lay == kElemLayout ? elem_node(sizeof(U) >= 4, n, ix, iu, iv)
                                              : (iv < 8 ? (ix * n + iu) * 8 + iv
                                                        : 8 * n * n + (ix * n + iu) * (n - 8) + (iv - 8));
         const int64_t e = ex * sxe_e + ((int64_t)eu * (nv + 2) + ev) * ep + inner;
