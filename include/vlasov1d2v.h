@@ -131,9 +131,11 @@ int vl_state_elems(vl_geom_t h, size_t *elems);
 /* Layouts of a state array (no counterpart in the reference, which has only
  * the first): VL_LAYOUT_REFERENCE is the C-order layout above;
  * VL_LAYOUT_ELEMENT_MAJOR keeps the n^3 nodes of each element contiguous
- * ((ix, iu, iv) inside, elements ordered (ex, eu, ev)) with a pitch of
- * round_up(n^3, 8) values per element, so that a tile of elements is one
- * contiguous run.  It is the resident layout of Stepper.main_loop. */
+ * (elements ordered (ex, eu, ev)) with a pitch of round_up(n^3, 8) values
+ * per element, so that a tile of elements is one contiguous run.  Inside an
+ * element node (ix, iu, iv) sits at (iu n + iv) n + ix for VL_F64 / VL_F32
+ * states (x fastest) and at (ix n + iu) n + iv for VL_BF16 / VL_F16 states.
+ * It is the resident layout of Stepper.main_loop. */
 /* VL_LAYOUT_ROW_SPLIT (n = 8 or 9 only, states under 2^31 bytes): elements ordered and padded as in
  * the element-major layout; inside an element the first eight nodes of each
  * v-line (ix, iu) form one 16-byte row (rows in (ix, iu) order), and for

@@ -213,14 +213,19 @@ __device__ __forceinline__ double tile_column_sum(double *red, int tid) {
     return acc;
 }
 
-// Workgroups per CU the stage kernel is compiled for.  The element-major FP64
-// stage-2/3 kernel at n = 9 fits 4 per CU (40 KB of LDS, 128 registers).  The
+// Workgroups per CU the stage kernel is compiled for (a register budget; LDS
+// sets its own limit).  The element-major FP64 stage-2/3 kernel at n = 9 fits
+// 4 per CU (40 KB of LDS, 128 registers).  The element-major FP32 first stage
+// at n = 9 (28 KB of LDS, no f^n) fits 5 within 96 registers and runs 5 %
+// faster there than at 4 (its stages 2/3 spill at 96 and stay at 3).  The
 // others are compiled for 3: the reference-layout kernels would spill at 128,
-// and the FP64 first stage (no f^n to hide) runs 5 % faster with the larger
-// register budget at 3 per CU than at 4 (measured on MI355X).
+// and the FP64 first stage ran faster with the larger budget (measured on
+// MI355X).
 template <int N, int S, int L, int MODE = kModeStage>
 constexpr int stage_min_blocks() {
-    return (N == 9 && S == 0 && L == kElemLayout && MODE == kModeStage) ? 4 : 3;
+    if (N == 9 && S == 0 && L == kElemLayout && MODE == kModeStage) return 4;
+    if (N == 9 && S == 1 && L == kElemLayout && MODE == kModeStage1) return 5;
+    return 3;
 }
 
 template <int N, int MODE, int S, int L, bool ADT = false>
