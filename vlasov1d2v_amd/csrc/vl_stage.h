@@ -331,4 +331,61 @@ __global__ __launch_bounds__(kBlock) void k_convert_layout(const U *__restrict__
     }
 }
 
+// Layout conversion reference <-> element-major for the x-fastest node order
+// of the FP64 / FP32 states, staged through LDS so that both sides move
+// contiguous runs: one workgroup per (ex, ix-free, eu, iu) slab walks the v
+// elements in chunks of kCvtEv.  The reference side of a chunk is n rows
+// (ix = 0 .. n-1) of kCvtEv n consecutive values; the element side is, per
+// v element, the n^2 values (iv, ix) of plane iu, one contiguous run.
+// Grid: (nx + 2) (nu + 2) n workgroups.  Padding values are not written.
+constexpr int kCvtEv = 8;
+template <typename U, int N>
+__global__ __launch_bounds__(kBlock) void k_convert_xminor(const U *__restrict__ src, U *__restrict__ dst,
+                                                           int to_elem, int nu, int nv) {
+    constexpr int RL = kCvtEv * N + 1;            // LDS row (one value of padding: bank spread)
+    constexpr int EP = elem_pitch(N);
+    __shared__ U buf[N * RL];
+    const int b = blockIdx.x;
+    const int iu = b % N;
+    const int t2 = b / N;
+    const int eu = t2 % (nu + 2);
+    const int ex = t2 / (nu + 2);
+    const int64_t vrow = (int64_t)(nv + 2) * N;                      // values of one (ix, eu, iu) row
+    const int64_t su = vrow;                                         // ref strides
+    const int64_t seu = su * N, six = seu * (nu + 2);
+    const int64_t ref0 = (int64_t)ex * N * six + (int64_t)eu * seu + (int64_t)iu * su;   // ix = 0, ev = 0
+    const int64_t el0 = (((int64_t)ex * (nu + 2) + eu) * (nv + 2)) * EP + (int64_t)iu * N * N;
+    for (int ev0 = 0; ev0 < nv + 2; ev0 += kCvtEv) {
+        const int nev = min(kCvtEv, nv + 2 - ev0);
+        if (to_elem) {
+            for (int idx = threadIdx.x; idx < N * kCvtEv * N; idx += kBlock) {
+                const int ix = idx / (kCvtEv * N), q = idx - ix * (kCvtEv * N);
+                if (q < nev * N) buf[ix * RL + q] = src[ref0 + ix * six + (int64_t)ev0 * N + q];
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < kCvtEv * N * N; idx += kBlock) {
+                const int e = idx / (N * N), r = idx - e * (N * N);      // r = iv n + ix
+                if (e < nev) {
+                    const int iv = r / N, ix = r - iv * N;
+                    dst[el0 + (int64_t)(ev0 + e) * EP + r] = buf[ix * RL + e * N + iv];
+                }
+            }
+        } else {
+            for (int idx = threadIdx.x; idx < kCvtEv * N * N; idx += kBlock) {
+                const int e = idx / (N * N), r = idx - e * (N * N);
+                if (e < nev) {
+                    const int iv = r / N, ix = r - iv * N;
+                    buf[ix * RL + e * N + iv] = src[el0 + (int64_t)(ev0 + e) * EP + r];
+                }
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < N * kCvtEv * N; idx += kBlock) {
+                const int ix = idx / (kCvtEv * N), q = idx - ix * (kCvtEv * N);
+                if (q < nev * N) dst[ref0 + ix * six + (int64_t)ev0 * N + q] = buf[ix * RL + q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace vl

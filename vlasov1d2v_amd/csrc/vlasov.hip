@@ -346,6 +346,17 @@ hipError_t launch_moment_rs_ns(const vl_geom *h, const void *f, const StageArgs 
     return hipGetLastError();
 }
 
+template <typename U, int N>
+hipError_t launch_convert_xminor_n(const vl_geom *h, dim3 grid, const U *src, U *dst, int te, hipStream_t st) {
+    hipLaunchKernelGGL((k_convert_xminor<U, N>), grid, dim3(kBlock), 0, st, src, dst, te, h->nu, h->nv);
+    return hipGetLastError();
+}
+
+template <typename U>
+hipError_t convert_xminor(const vl_geom *h, dim3 grid, const U *src, U *dst, int te, hipStream_t st) {
+    VL_DISPATCH_N((launch_convert_xminor_n<U, NN>(h, grid, src, dst, te, st)))
+}
+
 hipError_t launch_moment(const vl_geom *h, int storage, int layout, const void *f, const StageArgs &s,
                          hipStream_t st) {
     if (layout == kRowSplitLayout) {
@@ -1106,6 +1117,16 @@ int vl_convert_layout(vl_geom_t h, const void *src, int src_layout, void *dst, i
     const int64_t blocks = std::min<int64_t>((total + kBlock - 1) / kBlock, 65536);
     hipStream_t st = (hipStream_t)stream;
     const int te = to_elem ? 1 : 0;
+    if (lay == VL_LAYOUT_ELEMENT_MAJOR && (eb == 8 || eb == 4)) {
+        // x-fastest element order (elem_x_minor): slab-wise through LDS
+        const dim3 grid((unsigned)((h->nx + 2) * (h->nu + 2) * h->n));
+        if (eb == 8) {
+            VL_HIP(convert_xminor<double>(h, grid, (const double *)src, (double *)dst, te, st));
+        } else {
+            VL_HIP(convert_xminor<float>(h, grid, (const float *)src, (float *)dst, te, st));
+        }
+        return VL_OK;
+    }
     if (eb == 8)
         hipLaunchKernelGGL(k_convert_layout<double>, dim3(blocks), dim3(kBlock), 0, st, (const double *)src,
                            (double *)dst, te, lay, h->n, h->nu, h->nv, total);
