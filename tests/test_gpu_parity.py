@@ -317,6 +317,27 @@ def test_resident_steps_every_order_match_oracle(cuda, n, nx, nu, nv):
     assert err_f < 1e-12 and err_e < 1e-12
 
 
+@pytest.mark.parametrize("xseg", [1, 2, 3, 5])
+def test_pencil_walks_match_oracle(cuda, xseg):
+    """The stage kernel walks each pencil along x in the direction of its
+    x-line speeds (towards -x for the u < 0 elements, +x for u > 0, +x with a
+    per-tile x-plane for the middle element of an odd nu, which holds both
+    signs).  x-segments of 1, 2, 3 and 5 elements on nx = 5 (partial last
+    segments, periodic wrap at both ends of each walk) give the oracle's
+    resident FP64 steps to 1e-12."""
+    from vlasov1d2v_amd import _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    n, nx, nu, nv = 4, 5, 3, 4
+    case, run = make_pair(n, nx, nu, nv, dt=0.002)
+    run.grids.geometry(STORAGE_CODES["f64"]).set_tuning(_lib.VL_TUNE_XSEG, xseg)
+    st_ref = case.run(3, 0.002)
+    st = run.run(3)
+    err_f = rel_l2(interior(run.dist.arr.cpu().numpy()), interior(case.dist.arr))
+    err_e = rel_l2(st.field_energy, st_ref.field_energy)
+    print(f"xseg={xseg}: rel-L2 f {err_f:.2e}, energy {err_e:.2e}", flush=True)
+    assert err_f < 1e-12 and err_e < 1e-12
+
+
 @pytest.mark.parametrize("n", [4, 9, 10])
 def test_rhs_with_the_reference_volume_operator(cuda, n):
     """A geometry built from the reference's own up table (scipy polynomials
