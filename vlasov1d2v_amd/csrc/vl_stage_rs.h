@@ -24,7 +24,7 @@
 // permutation of that lane's operand.  The operator is split into hi + lo
 // 16-bit parts (two MFMAs into one FP32 accumulator).
 //
-// Brick.  Each wave owns one v-element of the tile and keeps it in LDS as a
+// Brick.  Each wave owns one element of the tile and keeps it in LDS as a
 // brick of 12 x-slots x 12 u-slots x 12 v-slots (the slot convention above
 // on every axis: the face slots 10/11 of one axis hold the neighbour
 // element's face planes).  One 8-byte LDS read gives a lane four v-slots of
@@ -43,8 +43,9 @@
 // the background images of the reduced forms, scale * A+-_u f0 and
 // scale * A_v f0, are the accumulator inputs of those MFMAs.
 //
-// Work decomposition.  A workgroup of 4 waves owns a pencil (eu, 4
-// consecutive v-elements, x-segment) and walks it along x.  The waves are
+// Work decomposition.  A workgroup of 4 waves owns a pencil (4 consecutive
+// elements of the velocity plane in (eu, ev) order, x-segment) and walks it
+// along x (decode_pencil_rs).  The waves are
 // independent (no barriers): each writes the moment partial of its own
 // element, one slot per tile element.
 // The global loads of the next tile are in flight while the current one is
@@ -81,10 +82,38 @@ __device__ __forceinline__ void task_fence() {
 #endif
 }
 
+// Work decomposition of the row-split kernels.  A tile is (ex, group g of TV
+// consecutive elements of the velocity plane in (eu, ev) order), one wave per
+// element, so that no wave idles when nv is not a multiple of TV (the groups
+// of a plane row may reach into the next row; only the plane's last group
+// can be partial).  s.nchunk holds the number of groups per x element; a
+// pencil is (g, x-segment).
+__device__ __forceinline__ Pencil decode_pencil_rs(const StageArgs &s, int p) {
+    Pencil c;
+    c.chunk = p % s.nchunk;
+    const int seg = p / s.nchunk;
+    c.eu = 0;                               // per wave: rs_wave_element
+    c.ex_begin = 1 + seg * s.xseg;
+    c.ex_end = min(s.nx, c.ex_begin + s.xseg - 1);
+    return c;
+}
+
+// Element (eu, ev) of wave el in group g; false past the plane's last
+// element (then (eu, ev) is the last element, so that addresses stay valid).
+template <int TV>
+__device__ __forceinline__ bool rs_wave_element(const StageArgs &s, int g, int el, int &eu, int &ev) {
+    const int plane = s.nu * s.nv;
+    const int q = g * TV + el;
+    const int qc = min(q, plane - 1);
+    eu = 1 + qc / s.nv;
+    ev = 1 + qc - (qc / s.nv) * s.nv;
+    return q < plane;
+}
+
 template <int N>
 struct RsShape {
     static_assert(N == 8 || N == 9, "row-split stage kernel: 8 or 9 nodes per axis");
-    static constexpr int TV = 4;                  // v-elements per tile, one wave each
+    static constexpr int TV = 4;                  // elements per tile, one wave each
     static constexpr int EP = elem_pitch(N);
     static constexpr int ROWS = N * N;            // 8-value rows per element
     static constexpr int NRI = (ROWS + 63) / 64;  // rows per lane
@@ -377,12 +406,9 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const int el = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's element (wave-uniform)
     const int lj = ln & 15;             // MFMA column / row within a 16 x 16 block
     const int lg = ln >> 4;             // lane group: K slots / output rows 4 lg .. 4 lg + 3
-    const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
-    const int eu = pc.eu;
-    const int v0 = 1 + pc.chunk * TV;
-    const int nvalid = min(TV, s.nv + 1 - v0);
-    const int ev = v0 + el;
-    const bool active = el < nvalid;    // wave-uniform
+    const Pencil pc = decode_pencil_rs(s, xcd_remap(blockIdx.x, gridDim.x));
+    int eu, ev;
+    const bool active = rs_wave_element<TV>(s, pc.chunk, el, eu, ev);   // wave-uniform
     const int vrow = s.nv + 2;
     const int64_t sxe = s.sxe;
     unsigned int *bk = brick + el * ESD;
@@ -558,7 +584,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     };
 
     const float c0 = (float)s.c0, c1 = (float)s.c1, c2 = (float)s.c2dt;
-    const int tiles_per_ex = s.nu * s.nchunk;
+    const int tiles_per_ex = s.nchunk;      // element groups per x element
     const bool partials = s.write_partials != 0;
     const vl_f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
 
@@ -736,7 +762,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             }
         }
         // moment partial of this wave's element (one slot per tile element,
-        // zero for the unused elements of a partial chunk); no barrier
+        // zero for the unused elements of the plane's last group); no barrier
         if (partials) {
             wave_sync_lds();
             int lno = ln;
@@ -746,7 +772,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
             const int rix = lno >> 2;
             if ((lno & 3) == 0 && rix < N) {
                 const int64_t slot =
-                    (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + rix;
+                    (((int64_t)(ex - 1) * tiles_per_ex + pc.chunk) * TV + el) * N + rix;
                 s.partials[slot] = make_double2(active ? (double)a * s.inv_scale : 0.0, 0.0);
             }
         }
@@ -757,8 +783,8 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
 // Moment partials of a row-split state (first stage of a loop), in the
 // partial layout of k_stage_rs (one slot per tile element) and with its
 // arithmetic (rs_unit_moment, rs_element_moment), so that both give the same
-// partials for the same stored values: one workgroup per tile (ex, eu,
-// 4 v-elements), one wave per element.
+// partials for the same stored values: one workgroup per tile (ex, group of
+// 4 elements), one wave per element.
 template <int N, int S>
 __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict__ f, StageArgs s,
                                                       const OpTables *__restrict__ tabp) {
@@ -768,18 +794,15 @@ __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict
     const int tid = threadIdx.x, ln = tid & 63, el = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int chunk = tile % s.nchunk;
-    const int t2 = tile / s.nchunk;
-    const int eu = 1 + t2 % s.nu;
-    const int ex = 1 + t2 / s.nu;
-    const int v0 = 1 + chunk * TV;
-    const int nvalid = min(TV, s.nv + 1 - v0);
+    const int ex = 1 + tile / s.nchunk;
+    int eu, ev;
+    const bool active = rs_wave_element<TV>(s, chunk, el, eu, ev);
     __shared__ __attribute__((aligned(16))) float wtab[kRsWeights];
     rs_weights_fill(wtab, *(ConstOpTables *)tabp, tid);
     __syncthreads();
     float *rw = rwk + el * RW;
-    const bool active = el < nvalid;
     if (active) {
-        const int64_t b = (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + v0 + el) * EP;
+        const int64_t b = (int64_t)ex * s.sxe + ((int64_t)eu * (s.nv + 2) + ev) * EP;
 #pragma unroll
         for (int p = 0; p < NPASS; ++p) {
             const int u0 = 64 * p + ln;
@@ -792,8 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_moment_rs(const uint16_t *__restrict
     const float a = rs_element_moment<N>(rw, ln, wtab, *(ConstOpTables *)tabp);
     const int rix = ln >> 2;
     if ((ln & 3) == 0 && rix < N) {
-        const int tiles_per_ex = s.nu * s.nchunk;
-        const int64_t slot = (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + chunk) * TV + el) * N + rix;
+        const int64_t slot = (((int64_t)(ex - 1) * s.nchunk + chunk) * TV + el) * N + rix;
         s.partials[slot] = make_double2(active ? (double)a * s.inv_scale : 0.0, 0.0);
     }
 }

@@ -45,7 +45,7 @@ namespace vl {
 template <int N>
 struct Rs2Shape {
     static_assert(N == 9, "second-layout row-split kernel: 9 nodes per axis");
-    static constexpr int TV = 4;                      // v-elements per tile, one wave each
+    static constexpr int TV = 4;                      // elements per tile, one wave each
     static constexpr int EP = elem_pitch(N);
     static constexpr int ROWS = N * N;                // rows (= v-lines) per element
     static constexpr int NRI = (ROWS + 63) / 64;      // own rows per lane (loads)
@@ -113,12 +113,9 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
     const int el = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c16 = ln & 15;            // product column
     const int g = ln >> 4;              // K group (operands) / row group (results)
-    const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
-    const int eu = pc.eu;
-    const int v0 = 1 + pc.chunk * TV;
-    const int nvalid = min(TV, s.nv + 1 - v0);
-    const int ev = v0 + el;
-    const bool active = el < nvalid;
+    const Pencil pc = decode_pencil_rs(s, xcd_remap(blockIdx.x, gridDim.x));
+    int eu, ev;
+    const bool active = rs_wave_element<TV>(s, pc.chunk, el, eu, ev);
     const int vrow = s.nv + 2;
     const int64_t sxe = s.sxe;
     unsigned int *bk = brick + el * SH::BRICK;
@@ -325,7 +322,7 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
         }
     };
 
-    const int tiles_per_ex = s.nu * s.nchunk;
+    const int tiles_per_ex = s.nchunk;      // element groups per x element
     const bool partials = s.write_partials != 0;
     const vl_f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
     const float jua = (float)(s.c2dt * s.ju * s.acc);      // u speed offset per unit E
@@ -480,7 +477,7 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
                 if (active)
                     for (int iu = 0; iu < N; ++iu) a += (double)mr[ln * N + iu];
                 const int64_t slot =
-                    (((int64_t)(ex - 1) * tiles_per_ex + (eu - 1) * s.nchunk + pc.chunk) * TV + el) * N + ln;
+                    (((int64_t)(ex - 1) * tiles_per_ex + pc.chunk) * TV + el) * N + ln;
                 s.partials[slot] = make_double2(a * s.inv_scale, 0.0);
             }
         }

@@ -37,7 +37,7 @@ struct vl_geom {
     int xseg, pencils;     // stage kernel: x elements per pencil segment, pencils launched
     int mfma;              // 1: matrix-core stage kernel for the 16-bit forms (element-major)
     int row_split;         // 1: the 16-bit forms with n = 8, 9 prefer the row-split layout
-    int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: 4 v-elements per tile
+    int nchunk4, tiles_per_ex4, tiles4, pencils4;   // row-split kernels: groups of 4 plane elements per tile
     int xseg4;             // row-split stage kernel: x elements per pencil
     int rounding;          // 16-bit forms, row-split kernel: 0 nearest-even, 1 stochastic
     int rs_gen;            // row-split stage kernel at n = 9: 1 k_stage_rs, 2 k_stage_rs2
@@ -730,12 +730,13 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
         h->mfma = 1;
         h->pencils = base * ((d->nx + h->xseg - 1) / h->xseg);
     }
-    // row-split kernels: tiles of 4 v-elements (same x-segments)
+    // row-split kernels: tiles of 4 consecutive elements of the velocity
+    // plane in (eu, ev) order (decode_pencil_rs), one wave each
     h->row_split = 1;
     h->rounding = 1;
     h->rs_gen = 1;
-    h->nchunk4 = (d->nv + 3) / 4;
-    h->tiles_per_ex4 = d->nu * h->nchunk4;
+    h->nchunk4 = (d->nu * d->nv + 3) / 4;
+    h->tiles_per_ex4 = h->nchunk4;
     h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
     // about 5200 workgroups (8 x elements per pencil at 116.6M DOF: measured
     // best among 4, 8, 16 on MI355X)
