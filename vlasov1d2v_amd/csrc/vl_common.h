@@ -206,6 +206,7 @@ struct StageArgs {
     int nx, nu, nv;                // interior elements
     int nchunk;                    // v-chunks per (ex, eu)
     int xseg;                      // x elements per pencil segment (stage kernel)
+    int reverse;                   // 1: stage kernels take their pencils last to first
     int periodic_x;                // 1: x-neighbours taken periodically from the interior
     int write_partials;            // 1: emit moment partials of the written values
     double jx, ju, jv;             // Jacobians 2/dx
@@ -254,6 +255,13 @@ __device__ __forceinline__ int xcd_remap(int b, int G) {
     const int rem = G % kNumXcd;
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
+
+// Logical pencil of this workgroup of a stage kernel: the XCD-aware remap,
+// and with s.reverse the pencils taken last to first.  The RK stages
+// alternate the order, so that each stage starts on the pencils the previous
+// one wrote last, which the device's last-level cache may still hold.
+#define VL_STAGE_PENCIL(s) ((s).reverse ? (int)gridDim.x - 1 - xcd_remap(blockIdx.x, gridDim.x) \
+                                        : xcd_remap(blockIdx.x, gridDim.x))
 
 // Error-free transformation a + b = s + e (Knuth two-sum).  Products that
 // feed it are rounded separately (no contraction in the callers).

@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock, (stage_min_blocks<N, S, L, MODE>())) void k
     __shared__ double bgs[StageLds<N, S>::n_bgs];
     __shared__ double red_own[StageLds<N, S>::n_red];
     if constexpr (ADT) stage_dt_from_device(s);   // adaptive runs: dt from the device clock
-    const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
+    const Pencil pc = decode_pencil(s, VL_STAGE_PENCIL(s));
     const int stage_tid = threadIdx.x;
 #include "vl_stage_body.inc"
 }

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_stage_mx(
     const int wave = tid >> 6;
     const int lj = ln & 15;            // MFMA column (line within the group)
     const int lg = ln >> 4;            // MFMA K / row group
-    const Pencil pc = decode_pencil(s, xcd_remap(blockIdx.x, gridDim.x));
+    const Pencil pc = decode_pencil(s, VL_STAGE_PENCIL(s));
     const int eu = pc.eu;
     const int v0 = 1 + pc.chunk * TV;
     const int nvalid = min(TV, s.nv + 1 - v0);

@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kBlock, VL_RS_WAVES) void k_stage_rs(
     const int el = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's element (wave-uniform)
     const int lj = ln & 15;             // MFMA column / row within a 16 x 16 block
     const int lg = ln >> 4;             // lane group: K slots / output rows 4 lg .. 4 lg + 3
-    const Pencil pc = decode_pencil_rs(s, xcd_remap(blockIdx.x, gridDim.x));
+    const Pencil pc = decode_pencil_rs(s, VL_STAGE_PENCIL(s));
     int eu, ev;
     const bool active = rs_wave_element<TV>(s, pc.chunk, el, eu, ev);   // wave-uniform
     const int vrow = s.nv + 2;

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock, VL_RS2_WAVES) void k_stage_rs2(
     const int el = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c16 = ln & 15;            // product column
     const int g = ln >> 4;              // K group (operands) / row group (results)
-    const Pencil pc = decode_pencil_rs(s, xcd_remap(blockIdx.x, gridDim.x));
+    const Pencil pc = decode_pencil_rs(s, VL_STAGE_PENCIL(s));
     int eu, ev;
     const bool active = rs_wave_element<TV>(s, pc.chunk, el, eu, ev);
     const int vrow = s.nv + 2;

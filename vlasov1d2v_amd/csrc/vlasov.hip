@@ -45,6 +45,7 @@ struct vl_geom {
     int64_t sx, sxe, su, sue;
     size_t state_elems;
     int device;
+    int flip;              // pencil order of the next step's first stage (alternates)
     double *d_xnodes, *d_unodes, *d_vnodes, *d_wts, *d_kwave;
     double2 *d_T, *d_phases;
     double *d_bg;          // reduced forms: G+, G-, Hv background tables
@@ -965,6 +966,11 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
     s.c1 = 1.0;
     s.c2 = 1.0;
     s.c2dt = dt;
+    // pencil orders alternate from stage to stage and, three stages per step,
+    // from step to step (the results do not depend on the order)
+    const int flip = h->flip;
+    h->flip ^= 1;
+    s.reverse = flip;
     VL_HIP(record_stage_event(h, 0, st));
     VL_HIP(launch_stage<kModeStage1>(h, S, LAY, f, f, nullptr, f1, E, s, st));
     VL_HIP(record_stage_event(h, 1, st));
@@ -974,6 +980,7 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
     s.c1 = 1.0 / 4.0;
     s.c2 = 1.0 / 4.0;
     s.c2dt = s.c2 * dt;
+    s.reverse = flip ^ 1;
     VL_HIP(record_stage_event(h, 2, st));
     VL_HIP(launch_stage<kModeStage>(h, S, LAY, f1, f, f, f2, E, s, st));
     VL_HIP(record_stage_event(h, 3, st));
@@ -985,6 +992,7 @@ int rk3_step(vl_geom_t h, void *fv, void *f1v, void *f2v, double *E, double *phi
     s.c1 = 2.0 / 3.0;
     s.c2 = 2.0 / 3.0;
     s.c2dt = s.c2 * dt;
+    s.reverse = flip;
     VL_HIP(record_stage_event(h, 4, st));
     VL_HIP(launch_stage<kModeStage>(h, S, LAY, f2, f, f, f, E, s, st));
     VL_HIP(record_stage_event(h, 5, st));
