@@ -338,6 +338,33 @@ def test_pencil_walks_match_oracle(cuda, xseg):
     assert err_f < 1e-12 and err_e < 1e-12
 
 
+def _random_shapes(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(2, 11))
+        out.append((n, int(rng.integers(1, 6)), int(rng.integers(1, 6)), int(rng.integers(1, 8)),
+                    int(rng.integers(1, 4))))
+    return out
+
+
+@pytest.mark.parametrize("n,nx,nu,nv,xseg", _random_shapes(8, 2024))
+def test_resident_steps_random_shapes_match_oracle(cuda, n, nx, nu, nv, xseg):
+    """Seeded random small grids (n 2..10, one-element axes, odd and even nu,
+    partial v-chunks, x-segments that do and do not divide nx): resident FP64
+    steps against the oracle."""
+    from vlasov1d2v_amd import _lib
+    from vlasov1d2v_amd.grid import STORAGE_CODES
+    case, run = make_pair(n, nx, nu, nv, dt=0.002)
+    run.grids.geometry(STORAGE_CODES["f64"]).set_tuning(_lib.VL_TUNE_XSEG, xseg)
+    st_ref = case.run(2, 0.002)
+    st = run.run(2)
+    err_f = rel_l2(interior(run.dist.arr.cpu().numpy()), interior(case.dist.arr))
+    err_e = rel_l2(st.field_energy, st_ref.field_energy)
+    print(f"n={n} {nx}x{nu}x{nv} xseg={xseg}: rel-L2 f {err_f:.2e}, energy {err_e:.2e}", flush=True)
+    assert err_f < 1e-12 and err_e < 1e-12
+
+
 @pytest.mark.parametrize("n", [4, 9, 10])
 def test_rhs_with_the_reference_volume_operator(cuda, n):
     """A geometry built from the reference's own up table (scipy polynomials
