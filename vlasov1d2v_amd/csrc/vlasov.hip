@@ -739,10 +739,14 @@ int vl_geom_create(const vl_geom_desc *d, vl_geom_t *out) {
     h->nchunk4 = (d->nu * d->nv + 3) / 4;
     h->tiles_per_ex4 = h->nchunk4;
     h->tiles4 = (int)std::min<int64_t>((int64_t)d->nx * h->tiles_per_ex4, 2147483647LL);
-    // about 5200 workgroups (8 x elements per pencil at 116.6M DOF: measured
-    // best among 4, 8, 16 on MI355X)
+    // x-segments: a power of two of them, about 2600 workgroups (16 x elements
+    // per pencil, 4 segments, at 116.6M DOF: 0.5-1.5 % faster than 8 elements
+    // on MI355X; segment counts that leave a short last segment, e.g. 5 of 13,
+    // were slower)
     {
-        int nseg = (5200 + h->tiles_per_ex4 - 1) / h->tiles_per_ex4;
+        const int want = std::max(1, 2600 / std::max(1, h->tiles_per_ex4));
+        int nseg = 1;
+        while (2 * nseg <= want) nseg *= 2;
         nseg = std::max(1, std::min(nseg, d->nx));
         h->xseg4 = (d->nx + nseg - 1) / nseg;
     }
